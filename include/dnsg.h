@@ -1,0 +1,236 @@
+/*
+ * dnsg.h -- C-ABI of the MI355X-native grid-embedded D3Q19 channel-flow time step.
+ *
+ * This header is the drop-in seam between the C driver (dnsg-run) and the HIP
+ * kernels in libdnsg.so.  It replaces the per-step function table of the reference
+ * solver (DNS-solver-with-grid-embedding/src/fun_defs.h:4-36), whose functions take
+ * a POINTER state struct by value (definitions.h:161-229) and mutate its arrays.
+ * Here the state lives on the GPU behind an opaque handle; host buffers are only
+ * borrowed for the duration of a call and every function returns an int status.
+ *
+ * Axis convention (same as the reference): x streamwise (periodic), y spanwise
+ * (periodic), z wall-normal.  Host-side array layouts are the reference's file
+ * layouts without ghost cells:
+ *   coarse lattice     [Xc][Yc][zdc][19]   (lbf-c.NNNN halves, main:1264-1274)
+ *   fine band lattice  [Xf][Yf][zdf][19]   (lbf-slf / lbf-suf.NNNN, main:1276-1298)
+ *   interface planes   [3][Xc][Yc][23]     (lbf-lin / lbf-uin.NNNN, main:1310-1340)
+ * with Xc = xdv, Yc = ydv, Xf = GR*Xc, Yf = GR*Yc, zdf = zds + 1.
+ *
+ * No torch or HIP types appear in any signature: plain pointers and sizes only.
+ */
+#ifndef DNSG_H
+#define DNSG_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DNSG_ABI_VERSION 1
+
+/* Wall surface pattern (definitions.h:70-72 PSLIP/RSLIP/SSLIP). */
+enum dnsg_pattern {
+    DNSG_PATTERN_NONE  = 0, /* all no-slip walls (solid_init.c:46-64 only)          */
+    DNSG_PATTERN_RSLIP = 1, /* streamwise ridges   (slip_ridges_init.c:15-81)       */
+    DNSG_PATTERN_PSLIP = 2, /* square posts        (slip_posts_init.c:15-132)       */
+    DNSG_PATTERN_SSLIP = 3  /* transverse ridges   (slip_spanwise_init.c:15-80)     */
+};
+
+/* Body-force driving. */
+enum dnsg_force_mode {
+    DNSG_FORCE_FLUX  = 0, /* reference controller: G = ub - <rho u> (encalc2.c:129) */
+    DNSG_FORCE_CONST = 1  /* constant body force g_const (coarse lattice units)      */
+};
+
+/* Storage / arithmetic tier. */
+enum dnsg_precision {
+    DNSG_PREC_FP64 = 0,   /* plain path: f stored and collided in FP64               */
+    DNSG_PREC_FP32 = 1,   /* f - w*rho0 stored in FP32, FP32 collision               */
+    DNSG_PREC_FP16 = 2    /* f - w*rho0 stored in FP16, FP32 collision               */
+};
+
+/* Initial state for dnsg_init_state. */
+enum dnsg_init_kind {
+    DNSG_INIT_REST          = 0, /* '-s' mode: f = ftemp = w on nodes with s > -1 (init.c:15-56) */
+    DNSG_INIT_LAMINAR_NOISE = 1  /* rho = 1, u_x = u_max*4z(1-z) + seeded uniform noise, f = feq */
+};
+
+/* Status codes. */
+enum dnsg_status {
+    DNSG_OK        =  0,
+    DNSG_ERR_ARG   = -1, /* invalid argument or parameter set                       */
+    DNSG_ERR_HIP   = -2, /* HIP runtime error                                        */
+    DNSG_ERR_NAN   = -3, /* NaN in the wall shear stress (wshearstr.c:285-289)       */
+    DNSG_ERR_OOM   = -4, /* device allocation failed                                 */
+    DNSG_ERR_STATE = -5, /* call out of sequence                                     */
+    DNSG_ERR_IO    = -6  /* file read/write failed (driver-side helpers)             */
+};
+
+/*
+ * Input parameters.  Field names mirror the compile-time #defines of the reference
+ * (definitions.h:34-119); the trailing block holds runtime-only extensions that the
+ * reference cannot express.  One process drives one GPU, so XDIM = YDIM = 1.
+ */
+typedef struct dnsg_params {
+    /* definitions.h:34-41 */
+    double Rebs;          /* bulk Reynolds number on 2h (tau formulas main:447-448) */
+    double ub;            /* target bulk velocity (flux controller, encalc2.c:129)  */
+    int    zdv;           /* coarse wall-normal points incl. the 2 overlap layers   */
+    int    ydv;           /* coarse spanwise points                                  */
+    int    xdv;           /* coarse streamwise points                                */
+    int    TSTR;          /* starting time index (file numbering)                    */
+    int    TEND;          /* end time index                                          */
+    double CFL;           /* only defines utime (main:439-441)                       */
+    /* definitions.h:47,51 */
+    int    gratio;        /* GR: coarse/fine resolution ratio                        */
+    int    zds;           /* fine fluid layers per band (zdf = zds + 1)              */
+    /* definitions.h:58-62 (fine-cell units) */
+    int    gyplussize, wyplussize, gxplussize, wxplussize;
+    /* definitions.h:70-72 */
+    int    pattern;       /* enum dnsg_pattern                                       */
+    /* definitions.h:109 ITER: 0 = undefined (use (TEND-TSTR)*utime)                */
+    long   ITER;
+    /* ---- runtime-only extensions ---- */
+    int    force_mode;    /* enum dnsg_force_mode                                    */
+    double g_const;       /* body force for DNSG_FORCE_CONST, coarse lattice units   */
+    double tau_c;         /* > 0: override coarse tau (tau_f = GR*(tau_c-0.5)+0.5)   */
+    int    precision;     /* enum dnsg_precision                                     */
+    int    init_kind;     /* enum dnsg_init_kind                                     */
+    uint64_t seed;        /* counter-based PRNG seed for DNSG_INIT_LAMINAR_NOISE     */
+    double u_max;         /* centreline velocity of the initial parabola             */
+    double noise;         /* noise half-width as a fraction of u_max                 */
+} dnsg_params;
+
+/* Derived sizes and scalars (main:139-155, 439-469). */
+typedef struct dnsg_dims {
+    int    Xc, Yc, zdc;   /* coarse extents without ghosts                          */
+    int    Xf, Yf, zdf;   /* fine band extents without ghosts (zdf includes the wall) */
+    int    GR;
+    int    NZs;           /* total wall-normal fine points incl. both wall nodes     */
+    int    NZ;            /* statistics profile length 2*zdf + zdc - 4               */
+    long   utime;         /* steps per unit non-dimensional time                     */
+    long   up;            /* last step index of the time loop (loop runs 0..up)      */
+    double dt;            /* time-stamp increment of the per-step logs               */
+    double omega_c;       /* 1/tau_c (reference variable 'tauc', main:448-451)       */
+    double omega_f;       /* 1/tau_f (reference variable 'tauf')                     */
+    double tau_c, tau_f;
+    size_t n_coarse;      /* Xc*Yc*zdc*19 doubles per coarse array                   */
+    size_t n_band;        /* Xf*Yf*zdf*19 doubles per band array                     */
+    size_t n_plane;       /* 3*Xc*Yc*23 doubles per interface-level file             */
+} dnsg_dims;
+
+/* Per-step scalars the driver appends to its text logs. */
+typedef struct dnsg_diag {
+    double time;          /* TM + dt*ss                                             */
+    double force;         /* V.force[0] after encalc2 (coarse units)                */
+    double momentum;      /* <rho u_x> weighted volume mean (encalc2.c:109)         */
+    double density;       /* density.txt value (encalc2.c:142)                      */
+    double flowrate;      /* flowrate.txt value (encalc2.c:138)                     */
+    double str_up;        /* str-up.txt (wshearstr.c:280)                           */
+    double str_down;      /* str-do.txt (wshearstr.c:227)                           */
+    double stress;        /* w-str.txt  (wshearstr.c:284)                           */
+    double pgrad;         /* p-grad.txt (wshearstr.c:278)                           */
+} dnsg_diag;
+
+typedef struct dnsg_ctx dnsg_t;
+
+/* ---- parameters ---------------------------------------------------------- */
+
+/* Fill *p with the defaults of definitions.h:34-119 scaled to one GPU. */
+int dnsg_params_default(dnsg_params *p);
+
+/* Derive sizes and relaxation rates (main:139-155, 432-469); validates the
+ * pattern periods (the reference overflows when the extent is not a multiple of
+ * the period, slip_ridges_init.c:40-42). */
+int dnsg_dims_from_params(const dnsg_params *p, dnsg_dims *d);
+
+/* ---- lifetime ------------------------------------------------------------- */
+
+/* Allocate the device state (replaces the calloc block main:473-883).  The wall
+ * masks are built on creation (solid_init + slip_*_init, main:964-973). */
+int dnsg_create(const dnsg_params *p, dnsg_t **out);
+int dnsg_destroy(dnsg_t *h);
+const char *dnsg_last_error(const dnsg_t *h);
+int dnsg_get_dims(const dnsg_t *h, dnsg_dims *d);
+
+/* ---- state ---------------------------------------------------------------- */
+
+/* '-s' mode init (init.c:15-56) or the seeded laminar+noise state, including the
+ * three interface time levels sampled from it (xcomputations.c:558-625). */
+int dnsg_init_state(dnsg_t *h);
+
+/* '-p' mode read path (main:999-1183): upload the ftemp halves of lbf-c/slf/suf,
+ * the three time levels of lbf-lin/uin (newest first) and force.NNNN.  Any array
+ * pointer may be NULL to keep the current device contents. */
+int dnsg_upload_state(dnsg_t *h, const double *coarse, const double *lower,
+                      const double *upper, const double *lin3, const double *uin3,
+                      double force);
+
+/* Checkpoint write path (main:1262-1341): download the current state (the
+ * reference's ftemp arrays) in the file layouts above. */
+int dnsg_download_state(dnsg_t *h, double *coarse, double *lower, double *upper,
+                        double *lin3, double *uin3, double *force);
+
+/* ---- the time step ---------------------------------------------------------- */
+
+/* Advance n coarse steps with the fused kernel sequence.  One step reproduces the
+ * loop body main:1256-1375: encalc2, wshearstr, xcomputations, GR x
+ * (xfcomputations, tinterpolate_hrmt, derivative, ctfdtransfer_opt, fexchange),
+ * ftcdtransfer, exchange.  If diag != NULL it receives n records, one per step,
+ * with the scalars encalc2/wshearstr compute at the top of that step.  Returns
+ * DNSG_ERR_NAN (after filling the records up to the failing step) when the wall
+ * stress is NaN. */
+int dnsg_step(dnsg_t *h, int n, dnsg_diag *diag);
+
+/* Same as dnsg_step but without per-step host synchronisation; the diag records
+ * are copied back once at the end.  *elapsed_ms receives the HIP-event time of
+ * the n steps (device time, inputs resident in HBM). */
+int dnsg_run_timed(dnsg_t *h, int n, dnsg_diag *diag, double *elapsed_ms);
+
+/* ---- reference-granular entry points (one per fun_defs.h function) ---------- */
+
+/* encalc2.c:17-147 -> V.force[0] (flux mode) and the flowrate/density scalars. */
+int dnsg_encalc2(dnsg_t *h, long ss, dnsg_diag *d);
+/* wshearstr.c:15-169 -> str-up/str-do/w-str/p-grad scalars; DNSG_ERR_NAN on NaN. */
+int dnsg_wshearstr(dnsg_t *h, long ss, dnsg_diag *d);
+/* xcomputations.c:16-950: coarse collide-stream, time-level rotation, sampling. */
+int dnsg_xcomputations(dnsg_t *h);
+/* xfcomputations.c:16-1144: one fine substep on both bands. */
+int dnsg_xfcomputations(dnsg_t *h);
+/* tinterpolate_hrmt.c + derivative.c + bcucof_opt.c + ctfdtransfer_opt.c for
+ * substep iter (1..GR): fills the fine interface layers. */
+int dnsg_ctfdtransfer(dnsg_t *h, int iter);
+/* ftcdtransfer.c:16-118: fine -> coarse injection after the last substep. */
+int dnsg_ftcdtransfer(dnsg_t *h);
+/* exchange.c:16-25 / fexchange.c:15-28: swap current/next lattices. */
+int dnsg_exchange(dnsg_t *h);
+int dnsg_fexchange(dnsg_t *h);
+
+/* ---- reductions and snapshots ----------------------------------------------- */
+
+/* Global mass check (main:1193-1250): "Average Density in the channel". */
+int dnsg_mass(dnsg_t *h, double *avg_density);
+
+/* statistics.c:15-379: instantaneous plane statistics, 9*NZ doubles in the order
+ * of turb-fld.000.NNN: U, V, W, u_rms, v_rms, w_rms, u'w', rho, rho_rms. */
+int dnsg_statistics(dnsg_t *h, double *out);
+
+/* Running (time-averaged) statistics accumulated on the device in FP64: adds the
+ * current plane means and second moments to the accumulators. */
+int dnsg_stats_accumulate(dnsg_t *h);
+/* Returns the time averages: 9*NZ doubles as above (U, V, W, u_rms, v_rms, w_rms,
+ * u'w', rho, rho_rms) plus *count samples. */
+int dnsg_stats_average(dnsg_t *h, double *out, long *count);
+int dnsg_stats_reset(dnsg_t *h);
+
+/* vel_print.c:15-274: velocity and density of one grid; grid 0 = coarse,
+ * 1 = lower band, 2 = upper band; vel is [X][Y][Z][3], den is [X][Y][Z]. */
+int dnsg_velocity_snapshot(dnsg_t *h, int grid, double *vel, double *den);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DNSG_H */

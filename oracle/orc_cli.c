@@ -1,0 +1,116 @@
+/*
+ * orc_cli.c -- command-line front end of the oracle (TEST INFRASTRUCTURE).
+ *
+ *   orc_cli init  <dir> key=value...   write the initial state as '-p' restart files
+ *                                      (lbf-c/slf/suf/lin/uin/force .TSTR) into dir
+ *   orc_cli run   <dir> key=value...   read the TSTR restart files from dir and run the
+ *                                      reference driver loop (main:1254-1375), writing
+ *                                      the same logs, turb-fld and lbf-* files
+ *   orc_cli masks <file> key=value...  dump the reference-layout node codes (int32)
+ *
+ * Keys are the dnsg_params field names (Rebs, ub, xdv, ydv, zdv, gratio, zds, CFL,
+ * TSTR, TEND, ITER, pattern, gyplussize, ... , seed, u_max, noise, init_kind,
+ * force_mode, g_const, tau_c, threads).
+ */
+#define _POSIX_C_SOURCE 200809L
+#include "dnsg_oracle.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static int set_param(dnsg_params *p, const char *kv, int *threads) {
+    char key[64];
+    const char *eq = strchr(kv, '=');
+    const char *val;
+    size_t n;
+    if (!eq) return -1;
+    n = (size_t)(eq - kv);
+    if (n >= sizeof key) return -1;
+    memcpy(key, kv, n);
+    key[n] = 0;
+    val = eq + 1;
+#define D(name) if (!strcmp(key, #name)) { p->name = strtod(val, NULL); return 0; }
+#define I(name) if (!strcmp(key, #name)) { p->name = (int)strtol(val, NULL, 10); return 0; }
+#define L(name) if (!strcmp(key, #name)) { p->name = strtol(val, NULL, 10); return 0; }
+    D(Rebs) D(ub) I(zdv) I(ydv) I(xdv) I(TSTR) I(TEND) D(CFL) I(gratio) I(zds)
+    I(gyplussize) I(wyplussize) I(gxplussize) I(wxplussize) I(pattern) L(ITER)
+    I(force_mode) D(g_const) D(tau_c) I(precision) I(init_kind) D(u_max) D(noise)
+#undef D
+#undef I
+#undef L
+    if (!strcmp(key, "seed")) { p->seed = strtoull(val, NULL, 10); return 0; }
+    if (!strcmp(key, "threads")) { *threads = (int)strtol(val, NULL, 10); return 0; }
+    return -1;
+}
+
+static void defaults(dnsg_params *p) {
+    memset(p, 0, sizeof *p);
+    p->Rebs = 7200.; p->ub = 0.0866034; p->zdv = 32; p->ydv = 32; p->xdv = 64;
+    p->TSTR = 0; p->TEND = 1; p->CFL = 0.1; p->gratio = 2; p->zds = 8;
+    p->gyplussize = 56; p->wyplussize = 8; p->gxplussize = 4; p->wxplussize = 4;
+    p->pattern = DNSG_PATTERN_NONE; p->ITER = 0; p->force_mode = DNSG_FORCE_FLUX;
+    p->init_kind = DNSG_INIT_LAMINAR_NOISE; p->seed = 1234; p->u_max = 0.05; p->noise = 0.05;
+}
+
+int main(int argc, char **argv) {
+    dnsg_params p;
+    orc_t *o;
+    char err[256] = "";
+    int i, threads = 0, rc = 0;
+    if (argc < 3) {
+        fprintf(stderr, "usage: %s init|run|masks <dir|file> key=value...\n", argv[0]);
+        return 2;
+    }
+    defaults(&p);
+    for (i = 3; i < argc; i++)
+        if (set_param(&p, argv[i], &threads)) {
+            fprintf(stderr, "bad parameter '%s'\n", argv[i]);
+            return 2;
+        }
+    if (threads > 0) orc_set_threads(threads);
+    o = orc_create(&p, err, sizeof err);
+    if (!o) {
+        fprintf(stderr, "orc_create: %s\n", err);
+        return 2;
+    }
+    if (!strcmp(argv[1], "init")) {
+        if (p.init_kind == DNSG_INIT_REST) orc_init_rest(o);
+        else orc_init_seeded(o);
+        if (p.force_mode == DNSG_FORCE_FLUX) orc_set_force(o, 0.);
+        rc = orc_write_checkpoint(o, argv[2], p.TSTR);
+    } else if (!strcmp(argv[1], "run")) {
+        dnsg_dims d;
+        orc_get_dims(o, &d);
+        rc = orc_read_checkpoint(o, argv[2], p.TSTR);
+        if (!rc) {
+            printf("Average Density in the channel at start: %.12f\n", orc_mass(o));
+            rc = orc_run(o, argv[2], d.up, 1);
+            printf("Average Density in the channel at end: %.12f\n", orc_mass(o));
+        }
+    } else if (!strcmp(argv[1], "masks")) {
+        dnsg_dims d;
+        int *sc, *sl, *su;
+        size_t nc, nf;
+        FILE *f;
+        orc_get_dims(o, &d);
+        nc = (size_t)(d.Xc + 1) * (d.Yc + 1) * d.zdc;
+        nf = (size_t)(d.Xf + 1) * (d.Yf + 1) * d.zdf;
+        sc = malloc(nc * sizeof(int)); sl = malloc(nf * sizeof(int)); su = malloc(nf * sizeof(int));
+        orc_ref_masks(o, sc, sl, su);
+        f = fopen(argv[2], "wb");
+        if (!f) rc = DNSG_ERR_IO;
+        else {
+            fwrite(sc, sizeof(int), nc, f);
+            fwrite(sl, sizeof(int), nf, f);
+            fwrite(su, sizeof(int), nf, f);
+            fclose(f);
+        }
+        free(sc); free(sl); free(su);
+    } else {
+        fprintf(stderr, "unknown command %s\n", argv[1]);
+        rc = 2;
+    }
+    orc_destroy(o);
+    return rc ? 1 : 0;
+}

@@ -1,0 +1,47 @@
+"""Pin the oracle to the reference solver.
+
+The fixtures in tests/golden/*.json were written by tests/golden/make_golden.py,
+which runs the reference (compiled from its own sources) on one MPI rank.  Each
+case feeds both programs the same '-p' restart files (or the reference's '-s'
+rest start) and compares, byte for byte, every checkpoint the reference wrote
+(both f/ftemp halves of lbf-c/slf/suf, the three interface time levels,
+force.NNNN), the last velocity/density snapshots, the statistics files, the
+output_twoD2 profile and the six per-step text logs.
+"""
+import glob
+import hashlib
+import json
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import GOLDEN, ORC_CLI
+
+CASES = sorted(glob.glob(os.path.join(GOLDEN, "*.json")))
+
+
+def _sha(path):
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[os.path.basename(c)[:-5] for c in CASES])
+def test_oracle_matches_reference(case, tmp_path, oracle_built):
+    fx = json.load(open(case))
+    args = [f"{k}={v}" for k, v in fx["params"].items()]
+    if fx["mode"] == "p":
+        args += [f"{k}={v}" for k, v in fx["init"].items()]
+    else:
+        args += ["init_kind=0"]
+    subprocess.run([ORC_CLI, "init", str(tmp_path)] + args, check=True)
+    out = subprocess.run([ORC_CLI, "run", str(tmp_path)] + args + ["threads=4"], check=True,
+                         capture_output=True, text=True).stdout
+    for name, digest in fx["sha256"].items():
+        assert _sha(tmp_path / name) == digest, f"{name} differs from the reference"
+    for name, text in fx["logs"].items():
+        assert (tmp_path / name).read_text() == text, f"{name} differs from the reference"
+    ref_mass = [re.search(r":\s*([-0-9.eE+]+)", l).group(1) for l in fx["mass"]]
+    got_mass = [re.search(r":\s*([-0-9.eE+]+)", l).group(1) for l in out.splitlines() if "Average" in l]
+    assert got_mass == ref_mass
