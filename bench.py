@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Benchmark of the grid-embedded D3Q19 channel time step on one MI355X.
+
+Workload (the Re_tau = 180 stripe channel): coarse 512 x 256 x 128 (x
+streamwise, y spanwise, z wall-normal, zdv = 128 including the two overlap
+layers), factor-2 wall bands of 16 fluid layers (fine 1024 x 512 x 17 each),
+streamwise superhydrophobic ridges on both walls (gas fraction 0.5, period 64
+fine = 32 coarse cells), u_tau = 0.005, nu = u_tau h / 180 with h the true
+half-height in coarse units ((NZs-2)/(2 GR) = 70.25), tau_c = 3 nu + 1/2,
+tau_f = 2 (tau_c - 1/2) + 1/2, constant body force u_tau^2 / h, laminar
+profile at the matching bulk velocity plus seeded +-10% noise.
+
+A step is one coarse time step: coarse collide-stream, GR = 2 band substeps
+with the interface transfers, the fine->coarse injection, and the per-step
+flux / density / wall-stress reductions the reference logs every step.
+LU per step = Nc + GR (N_lower + N_upper) = 50,331,648.
+
+value = FP64 plain-path GLUPS with the lattices resident in HBM.  The line also
+carries the shifted FP32 and FP16-storage tiers, the roofline of the dominant
+kernel (band collide-stream, measured live with HIP events on the solver's
+stream) and the oracle (C restatement, OpenMP) timed on a bounded sample on
+the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "tests"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "GLUPS + %HBM roofline, FP32 vs FP64 at Re_tau=180; p50/p99 slip-queue latency"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_LU = {"fp64": 304, "fp32": 152, "fp16": 76}
+
+
+def workload_params(mod, prec="fp64"):
+    GR, zds, zdv = 2, 16, 128
+    zdf = zds + 1
+    NZs = 2 * zdf + GR * (zdv - 4) + GR - 1
+    h = (NZs - 2.0) / (2.0 * GR)       # coarse units
+    u_tau = 0.005
+    nu = u_tau * h / 180.0
+    tau_c = 3.0 * nu + 0.5
+    G = u_tau * u_tau / h
+    ub = 15.6 * u_tau                   # bulk velocity of the laminar start (Dean: U_b ~ 15.6 u_tau)
+    return mod.make_params(xdv=512, ydv=256, zdv=zdv, gratio=GR, zds=zds, tau_c=tau_c, ub=ub,
+                           pattern="rslip", gyplussize=32, wyplussize=32, force_mode="const",
+                           g_const=G, precision=prec, init_kind="laminar_noise", seed=2024,
+                           u_max=1.5 * ub, noise=0.10, CFL=0.1)
+
+
+def lu_per_step(d):
+    return d.Xc * d.Yc * d.zdc + d.GR * 2 * d.Xf * d.Yf * (d.zdf - 1)
+
+
+def run_gpu(mod, prec, steps, warmup):
+    p = workload_params(mod, prec)
+    s = mod.Solver(p)
+    s.init_state()
+    if warmup:
+        s.run_timed(warmup)
+    t0 = time.perf_counter()
+    ms_ev, diag = s.run_timed(steps)   # synchronises the stream before returning
+    wall = time.perf_counter() - t0
+    kt = s.kernel_times(3)
+    d = s.dims
+    lu = lu_per_step(d)
+    out = dict(glups=lu * steps / wall / 1e9, ms_per_step=wall * 1e3 / steps, ms_per_step_events=ms_ev / steps,
+               kernels_ms=kt, lu_per_step=lu, flowrate_last=float(diag["flowrate"][-1]),
+               stress_last=float(diag["stress"][-1]))
+    band_lu = 2 * d.Xf * d.Yf * (d.zdf - 1)       # both bands, one launch
+    band_bytes = band_lu * BYTES_PER_LU[prec]
+    out["band_launch"] = dict(lu=band_lu, bytes=band_bytes, ms=kt["band"],
+                              gbs=band_bytes / (kt["band"] * 1e-3) / 1e9 if kt["band"] > 0 else None)
+    s.close()
+    return out
+
+
+def traffic_from_profiles():
+    """HBM bytes per band launch from the committed rocprofv3 --pmc summary, if any"""
+    path = os.path.join(REPO, "profiles", "pmc_band_collide_stream_retau180.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(mod, nsteps):
+    from helpers import Oracle  # the oracle: C restatement of the reference step
+    cores = min(16, int(os.environ.get("OMP_NUM_THREADS", "16")), os.cpu_count() or 16)
+    p = workload_params(mod, "fp64")
+    o = Oracle(p, threads=cores)
+    o.init_seeded()
+    o.step(0)  # first touch of every page outside the timed sample
+    t0 = time.perf_counter()
+    for s in range(1, nsteps + 1):
+        o.step(s)
+    dt = time.perf_counter() - t0
+    lu = lu_per_step(o.dims)
+    o.close()
+    return dict(value=lu * nsteps / dt / 1e9, unit="GLUPS", cores=cores, kind="port",
+                sample=f"{nsteps} coarse steps of the same 512x256x128 + 2x1024x512x17 channel "
+                       f"(oracle, C restatement of the reference step, OpenMP, {dt:.1f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tiers", default="fp64,fp32,fp16")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (the lattices of one channel live on a single "
+              "MI355X); multi-GPU runs are not part of it", file=sys.stderr)
+        sys.exit(2)
+    from helpers import dnsg as mod
+
+    tiers = [t for t in args.tiers.split(",") if t]
+    res = {}
+    for prec in tiers:
+        print(f"bench: {prec} {args.steps} steps (+{args.warmup} warmup)", file=sys.stderr, flush=True)
+        res[prec] = run_gpu(mod, prec, args.steps, args.warmup)
+        print(f"bench: {prec} {res[prec]['glups']:.3f} GLUPS, {res[prec]['ms_per_step']:.3f} ms/step, "
+              f"kernels {res[prec]['kernels_ms']}", file=sys.stderr, flush=True)
+    main_t = res["fp64"] if "fp64" in res else res[tiers[0]]
+    main_prec = "fp64" if "fp64" in res else tiers[0]
+    bl = main_t["band_launch"]
+    traffic = traffic_from_profiles()
+    roof = dict(bound="hbm", achieved=bl["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=(bl["gbs"] / HBM_PEAK_GBS) if bl["gbs"] else None, traffic=traffic,
+                kernel="k_collide_stream<double> (both wall bands, one launch per substep)",
+                algorithmic_bytes_per_launch=bl["bytes"], launch_ms=bl["ms"])
+    cpu = None
+    if not args.no_cpu:
+        print(f"bench: cpu baseline ({args.cpu_steps} steps)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(mod, args.cpu_steps)
+    line = {
+        "metric": METRIC,
+        "value": round(main_t["glups"], 4),
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_t["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16-storage/f32"}[main_prec],
+        "data": "synthetic (seeded laminar profile + 10% noise)",
+        "config": {"workload": "Re_tau=180 superhydrophobic-stripe channel, coarse 512x256x128, "
+                               "factor-2 bands 2x1024x512x17, 33.6M cells, constant body force",
+                   "lu_per_step": main_t["lu_per_step"], "parallelism": "single GPU"},
+        "hbm_roofline_frac_step": round(main_t["glups"] * BYTES_PER_LU[main_prec] / HBM_PEAK_GBS, 4),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "tiers": {k: {"glups": round(v["glups"], 4), "ms_per_step": round(v["ms_per_step"], 4),
+                      "hbm_frac": round(v["glups"] * BYTES_PER_LU[k] / HBM_PEAK_GBS, 4),
+                      "kernels_ms": {a: round(b, 4) for a, b in v["kernels_ms"].items()}}
+                  for k, v in res.items()},
+    }
+    if "fp64" in res and "fp32" in res:
+        line["fp32_over_fp64"] = round(res["fp32"]["glups"] / res["fp64"]["glups"], 3)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,831 @@
+// api.hip -- the dnsg_* C-ABI (include/dnsg.h): device state ownership, the
+// kernel sequence of one coarse step, uploads/downloads in the reference's file
+// layouts, and the reductions' host side.
+//
+// One handle = one GPU stream.  Every call is stream-ordered; calls that return
+// host data synchronise the stream.  There is no CPU fallback: every compute
+// path runs on the GPU or the call fails with DNSG_ERR_HIP.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <new>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace dnsg;
+
+#define HIPCHK(h, expr)                                                                           \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            (h)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                         \
+            return e_ == hipErrorOutOfMemory ? DNSG_ERR_OOM : DNSG_ERR_HIP;                       \
+        }                                                                                         \
+    } while (0)
+
+namespace {
+
+constexpr int NVP = 23;
+
+// parameter derivation, kept identical to main:139-155, 432-469
+int derive(const dnsg_params *p, dnsg_dims *d, std::string *err) {
+    memset(d, 0, sizeof(*d));
+    auto fail = [&](const char *m) {
+        if (err) *err = m;
+        return DNSG_ERR_ARG;
+    };
+    if (p->gratio < 1 || p->gratio > 4) return fail("gratio must be 1..4");
+    if (p->xdv < 2 || p->ydv < 2 || p->zdv < 5) return fail("xdv, ydv >= 2 and zdv >= 5 required");
+    d->GR = p->gratio;
+    d->Xc = p->xdv;
+    d->Yc = p->ydv;
+    d->zdc = p->zdv;
+    d->Xf = d->GR * d->Xc;
+    d->Yf = d->GR * d->Yc;
+    d->zdf = p->zds + 1;
+    if (d->zdf < 5 || d->zdf - 2 - d->GR < 0) return fail("zds must be >= max(4, gratio+1)");
+    d->NZs = 2 * d->zdf + d->GR * (d->zdc - 4) + d->GR - 1;
+    d->NZ = 2 * d->zdf + (d->zdc - 4);
+    if (!(p->CFL > 0.)) return fail("CFL must be positive");
+    const double ut = ((double)(0.5 * (d->NZs - 2.) / p->CFL)) / ((double)(d->GR));
+    d->utime = (long)(int)ut;
+    if (d->utime < 1) return fail("CFL gives fewer than one step per unit time");
+    d->dt = d->GR * 1. / ((double)(0.5 * (d->NZs - 2.) / p->CFL));
+    if (p->tau_c > 0.) {
+        d->tau_c = p->tau_c;
+        d->tau_f = d->GR * (p->tau_c - 0.5) + 0.5;
+    } else {
+        if (!(p->Rebs > 0.)) return fail("Rebs must be positive");
+        d->tau_f = (3. / p->Rebs) * p->ub * (d->NZs - 2.) + 0.5;
+        d->tau_c = (1. / ((double)d->GR)) * (3. / p->Rebs) * p->ub * (d->NZs - 2.) + 0.5;
+    }
+    if (!(d->tau_c > 0.5) || !(d->tau_f > 0.5)) return fail("relaxation times must exceed 1/2");
+    d->omega_f = 1. / d->tau_f;
+    d->omega_c = 1. / d->tau_c;
+    const long nut = p->TEND - p->TSTR;
+    d->up = (long)(nut * (double)d->utime);
+    if (p->ITER > 0) d->up = p->ITER;
+    d->n_coarse = (size_t)d->Xc * d->Yc * d->zdc * 19;
+    d->n_band = (size_t)d->Xf * d->Yf * d->zdf * 19;
+    d->n_plane = (size_t)3 * d->Xc * d->Yc * NVP;
+    if (p->pattern == DNSG_PATTERN_RSLIP || p->pattern == DNSG_PATTERN_PSLIP) {
+        const int P = p->gyplussize + p->wyplussize;
+        if (p->gyplussize < 0 || p->wyplussize < 0 || P <= 0 || d->Yf % P) return fail("gyplussize+wyplussize must divide the fine spanwise extent");
+    }
+    if (p->pattern == DNSG_PATTERN_SSLIP || p->pattern == DNSG_PATTERN_PSLIP) {
+        const int P = p->gxplussize + p->wxplussize;
+        if (p->gxplussize < 0 || p->wxplussize < 0 || P <= 0 || d->Xf % P) return fail("gxplussize+wxplussize must divide the fine streamwise extent");
+    }
+    if (p->pattern < 0 || p->pattern > 3) return fail("unknown pattern");
+    if (p->precision < 0 || p->precision > 2) return fail("unknown precision");
+    if (p->force_mode < 0 || p->force_mode > 1) return fail("unknown force mode");
+    return DNSG_OK;
+}
+
+// periodic stripe map of slip_ridges_init.c:37-42: solid (-1) except gas (-2)
+// on [w/2 + n(g+w), w/2 + n(g+w) + g)
+void stripe_map(int n, int gap, int width, std::vector<int> &g) {
+    g.assign(n, -1);
+    for (int j = width / 2; j < n; j += gap + width)
+        for (int a = 0; a < gap && j + a < n; a++) g[j + a] = -2;
+}
+
+// wall codes of both walls, [Yf][Xf] (solid_init.c:46-64 + slip_*_init.c)
+std::vector<int8_t> wall_codes(const dnsg_params *p, const dnsg_dims *d) {
+    std::vector<int> sx(d->Xf, -1), sy(d->Yf, -1);
+    if (p->pattern == DNSG_PATTERN_RSLIP) stripe_map(d->Yf, p->gyplussize, p->wyplussize, sy);
+    if (p->pattern == DNSG_PATTERN_SSLIP) stripe_map(d->Xf, p->gxplussize, p->wxplussize, sx);
+    if (p->pattern == DNSG_PATTERN_PSLIP) {
+        stripe_map(d->Xf, p->gxplussize, p->wxplussize, sx);
+        stripe_map(d->Yf, p->gyplussize, p->wyplussize, sy);
+    }
+    std::vector<int8_t> w((size_t)d->Xf * d->Yf);
+    for (int j = 0; j < d->Yf; j++)
+        for (int i = 0; i < d->Xf; i++) {
+            int c = -1;
+            if (p->pattern == DNSG_PATTERN_RSLIP) c = sy[j];
+            else if (p->pattern == DNSG_PATTERN_SSLIP) c = sx[i];
+            else if (p->pattern == DNSG_PATTERN_PSLIP) c = (sy[j] == -2) ? -2 : sx[i];
+            w[(size_t)j * d->Xf + i] = (int8_t)c;
+        }
+    return w;
+}
+
+int ensure_diag(dnsg_t *h, int n) {
+    if (n <= h->diag_cap) return DNSG_OK;
+    if (h->d_diag) HIPCHK(h, hipFree(h->d_diag));
+    h->d_diag = nullptr;
+    h->diag_cap = 0;
+    HIPCHK(h, hipMalloc(&h->d_diag, sizeof(double) * DIAG_WORDS * (size_t)n));
+    h->diag_cap = n;
+    return DNSG_OK;
+}
+
+int ensure_pinned(dnsg_t *h, size_t n) {
+    if (n <= h->h_pinned_cap) return DNSG_OK;
+    if (h->h_pinned) HIPCHK(h, hipHostFree(h->h_pinned));
+    h->h_pinned = nullptr;
+    h->h_pinned_cap = 0;
+    HIPCHK(h, hipHostMalloc((void **)&h->h_pinned, sizeof(double) * n, hipHostMallocDefault));
+    h->h_pinned_cap = n;
+    return DNSG_OK;
+}
+
+LatArgs coarse_args(dnsg_t *h) {
+    LatArgs L;
+    L.cur = h->coarse.c();
+    L.nxt = h->coarse.n();
+    L.nx = h->coarse.nx; L.ny = h->coarse.ny; L.nz = h->coarse.nz;
+    L.lo_kind = BOUND_DROP; L.hi_kind = BOUND_DROP;
+    L.wlo = nullptr; L.whi = nullptr;
+    L.gscale = 1.;
+    L.omega = h->d.omega_c;
+    return L;
+}
+
+void band_args(dnsg_t *h, LatArgs *L) {
+    const double gs = 1. / ((double)h->d.GR);  // xfcomputations' GR argument (main:1349)
+    for (int b = 0; b < 2; b++) {
+        Grid &g = b == 0 ? h->lower : h->upper;
+        L[b].cur = g.c();
+        L[b].nxt = g.n();
+        L[b].nx = g.nx; L[b].ny = g.ny; L[b].nz = g.nz;
+        L[b].lo_kind = b == 0 ? BOUND_WALL : BOUND_DROP;
+        L[b].hi_kind = b == 0 ? BOUND_DROP : BOUND_WALL;
+        L[b].wlo = b == 0 ? h->wall_lo : nullptr;
+        L[b].whi = b == 0 ? nullptr : h->wall_up;
+        L[b].gscale = gs;
+        L[b].omega = h->d.omega_f;
+    }
+}
+
+void rotate_levels(dnsg_t *h) {  // xcomputations.c:548-556
+    double *t = h->lev_lo[2];
+    h->lev_lo[2] = h->lev_lo[1]; h->lev_lo[1] = h->lev_lo[0]; h->lev_lo[0] = t;
+    t = h->lev_up[2];
+    h->lev_up[2] = h->lev_up[1]; h->lev_up[1] = h->lev_up[0]; h->lev_up[0] = t;
+}
+
+int do_xcomputations(dnsg_t *h) {
+    LatArgs L = coarse_args(h);
+    HIPCHK(h, launch_collide_stream(h->stream, &L, 1, h->d_force, h->p.precision));
+    rotate_levels(h);
+    HIPCHK(h, launch_sample(h->stream, h, h->coarse.n(), h->lev_lo[0], h->lev_up[0]));
+    return DNSG_OK;
+}
+
+int do_xfcomputations(dnsg_t *h) {
+    LatArgs L[2];
+    band_args(h, L);
+    HIPCHK(h, launch_collide_stream(h->stream, L, 2, h->d_force, h->p.precision));
+    return DNSG_OK;
+}
+
+// optional section timing: events recorded on the handle's stream between the
+// kernel groups of one step
+enum { SEC_REDUCE = 0, SEC_COARSE = 1, SEC_SAMPLE = 2, SEC_BAND = 3, SEC_IFACE = 4, SEC_N = 5 };
+struct SecTimer {
+    std::vector<hipEvent_t> ev;
+    std::vector<int> tag;  // section that ends at ev[i] (ev[0] starts the step)
+    void mark(hipStream_t s, int t) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        (void)hipEventRecord(e, s);
+        ev.push_back(e);
+        tag.push_back(t);
+    }
+};
+
+// one coarse step of main:1256-1375; rec receives the diag record on the device
+int do_step(dnsg_t *h, double *rec, SecTimer *T = nullptr) {
+    int rc;
+    if (T) T->mark(h->stream, -1);
+    HIPCHK(h, launch_encalc2(h->stream, h, rec));
+    HIPCHK(h, launch_wshear(h->stream, h, rec));
+    if (T) T->mark(h->stream, SEC_REDUCE);
+    {
+        LatArgs L = coarse_args(h);
+        HIPCHK(h, launch_collide_stream(h->stream, &L, 1, h->d_force, h->p.precision));
+        if (T) T->mark(h->stream, SEC_COARSE);
+        rotate_levels(h);
+        HIPCHK(h, launch_sample(h->stream, h, h->coarse.n(), h->lev_lo[0], h->lev_up[0]));
+        if (T) T->mark(h->stream, SEC_SAMPLE);
+    }
+    for (int iter = 1; iter <= h->d.GR; iter++) {
+        if ((rc = do_xfcomputations(h))) return rc;
+        if (T) T->mark(h->stream, SEC_BAND);
+        HIPCHK(h, launch_interface(h->stream, h, iter));
+        if (iter == h->d.GR) HIPCHK(h, launch_ftcd(h->stream, h));
+        if (T) T->mark(h->stream, SEC_IFACE);
+        h->lower.swap();
+        h->upper.swap();
+    }
+    h->coarse.swap();
+    h->ss++;
+    return DNSG_OK;
+}
+
+void fill_diag(const dnsg_t *h, const double *rec, long ss, dnsg_diag *d) {
+    d->time = ((double)h->p.TSTR + h->d.dt * ss);
+    d->force = rec[1];
+    d->momentum = rec[2];
+    d->density = rec[3];
+    d->flowrate = rec[4];
+    d->str_up = rec[5];
+    d->str_down = rec[6];
+    d->stress = rec[7];
+    d->pgrad = rec[8];
+}
+
+// [3][Xc][Yc][23] (file layout, newest first) <-> three device planes [23][Yc][Xc]
+int put_levels(dnsg_t *h, const double *lv, double **dev) {
+    const int Xc = h->d.Xc, Yc = h->d.Yc;
+    const size_t np = (size_t)Xc * Yc * NVP;
+    std::vector<double> t(np);
+    for (int l = 0; l < 3; l++) {
+        const double *src = lv + l * np;
+        for (int i = 0; i < Xc; i++)
+            for (int j = 0; j < Yc; j++)
+                for (int v = 0; v < NVP; v++)
+                    t[((size_t)v * Yc + j) * Xc + i] = src[((size_t)i * Yc + j) * NVP + v];
+        HIPCHK(h, hipMemcpyAsync(dev[l], t.data(), np * 8, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
+    return DNSG_OK;
+}
+
+int get_levels(dnsg_t *h, double *lv, double *const *dev) {
+    const int Xc = h->d.Xc, Yc = h->d.Yc;
+    const size_t np = (size_t)Xc * Yc * NVP;
+    std::vector<double> t(np);
+    for (int l = 0; l < 3; l++) {
+        HIPCHK(h, hipMemcpyAsync(t.data(), dev[l], np * 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        double *dst = lv + l * np;
+        for (int i = 0; i < Xc; i++)
+            for (int j = 0; j < Yc; j++)
+                for (int v = 0; v < NVP; v++)
+                    dst[((size_t)i * Yc + j) * NVP + v] = t[((size_t)v * Yc + j) * Xc + i];
+    }
+    return DNSG_OK;
+}
+
+// AoS [X][Y][zaos][19] host array <-> SoA device lattice (stored layers z0..z0+nz-1)
+int put_lattice(dnsg_t *h, const double *aos, Grid &g, int zaos, int z0) {
+    const size_t n = (size_t)g.nx * g.ny * zaos * 19;
+    double *stage = nullptr;
+    HIPCHK(h, hipMalloc(&stage, n * 8));
+    hipError_t e = hipMemcpyAsync(stage, aos, n * 8, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = launch_aos_to_soa(h->stream, stage, g.c(), g.nx, g.ny, zaos, z0, g.nz, h->p.precision);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(stage);
+    HIPCHK(h, e);
+    return DNSG_OK;
+}
+
+int get_lattice(dnsg_t *h, double *aos, const Grid &g, int zaos, int z0) {
+    const size_t n = (size_t)g.nx * g.ny * zaos * 19;
+    double *stage = nullptr;
+    HIPCHK(h, hipMalloc(&stage, n * 8));
+    hipError_t e = hipMemsetAsync(stage, 0, n * 8, h->stream);
+    if (e == hipSuccess) e = launch_soa_to_aos(h->stream, g.c(), stage, g.nx, g.ny, zaos, z0, g.nz, h->p.precision);
+    if (e == hipSuccess) e = hipMemcpyAsync(aos, stage, n * 8, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(stage);
+    HIPCHK(h, e);
+    return DNSG_OK;
+}
+
+// keep the wall layer k of a band AoS array on the host; returns its mass
+double keep_wall(const dnsg_t *h, const double *aos, int k, double *dst) {
+    const int Xf = h->d.Xf, Yf = h->d.Yf, zdf = h->d.zdf;
+    double m = 0.;
+    for (int i = 0; i < Xf; i++)
+        for (int j = 0; j < Yf; j++) {
+            const double *src = aos + (((size_t)i * Yf + j) * zdf + k) * 19;
+            double r = 0.;
+            for (int a = 0; a < 19; a++) {
+                dst[((size_t)i * Yf + j) * 19 + a] = src[a];
+                r += src[a];
+            }
+            m += r;
+        }
+    return m;
+}
+
+void place_wall(const dnsg_t *h, double *aos, int k, const double *src) {
+    const int Xf = h->d.Xf, Yf = h->d.Yf, zdf = h->d.zdf;
+    for (int i = 0; i < Xf; i++)
+        for (int j = 0; j < Yf; j++)
+            memcpy(aos + (((size_t)i * Yf + j) * zdf + k) * 19, src + ((size_t)i * Yf + j) * 19, 19 * 8);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dnsg_params_default(dnsg_params *p) {
+    if (!p) return DNSG_ERR_ARG;
+    memset(p, 0, sizeof(*p));
+    // definitions.h:34-62 with the grid of the 64x32x32 channel on one GPU
+    p->Rebs = 7200.000;
+    p->ub = 0.0866034;
+    p->zdv = 32;
+    p->ydv = 32;
+    p->xdv = 64;
+    p->TSTR = 0;
+    p->TEND = 1;
+    p->CFL = 0.1;
+    p->gratio = 2;
+    p->zds = 8;
+    p->gyplussize = 56;
+    p->wyplussize = 8;
+    p->gxplussize = 4;
+    p->wxplussize = 4;
+    p->pattern = DNSG_PATTERN_NONE;
+    p->ITER = 0;
+    p->force_mode = DNSG_FORCE_FLUX;
+    p->g_const = 0.;
+    p->tau_c = 0.;
+    p->precision = DNSG_PREC_FP64;
+    p->init_kind = DNSG_INIT_REST;
+    p->seed = 1234;
+    p->u_max = 0.05;
+    p->noise = 0.05;
+    return DNSG_OK;
+}
+
+int dnsg_dims_from_params(const dnsg_params *p, dnsg_dims *d) {
+    if (!p || !d) return DNSG_ERR_ARG;
+    return derive(p, d, nullptr);
+}
+
+const char *dnsg_last_error(const dnsg_t *h) { return h ? h->err.c_str() : "null handle"; }
+
+int dnsg_get_dims(const dnsg_t *h, dnsg_dims *d) {
+    if (!h || !d) return DNSG_ERR_ARG;
+    *d = h->d;
+    return DNSG_OK;
+}
+
+int dnsg_destroy(dnsg_t *h) {
+    if (!h) return DNSG_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    dnsg::Grid *gs[3] = {&h->coarse, &h->lower, &h->upper};
+    for (auto *g : gs)
+        for (int b = 0; b < 2; b++)
+            if (g->buf[b]) (void)hipFree(g->buf[b]);
+    for (int l = 0; l < 3; l++) {
+        if (h->lev_lo[l]) (void)hipFree(h->lev_lo[l]);
+        if (h->lev_up[l]) (void)hipFree(h->lev_up[l]);
+    }
+    void *ptrs[] = {h->slots_lo, h->slots_up, h->wall_lo, h->wall_up, h->part, h->d_force, h->d_diag, h->d_red, h->d_acc};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    if (h->h_pinned) (void)hipHostFree(h->h_pinned);
+    free(h->host_wall_lo);
+    free(h->host_wall_up);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return DNSG_OK;
+}
+
+int dnsg_create(const dnsg_params *p, dnsg_t **out) {
+    if (!p || !out) return DNSG_ERR_ARG;
+    *out = nullptr;
+    dnsg_t *h = new (std::nothrow) dnsg_ctx();
+    if (!h) return DNSG_ERR_OOM;
+    h->p = *p;
+    int rc = derive(p, &h->d, &h->err);
+    if (rc) {
+        *out = h;  // keep the handle so the caller can read the message
+        return rc;
+    }
+    *out = h;
+    HIPCHK(h, hipGetDevice(&h->device));
+    HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->elem = p->precision == DNSG_PREC_FP64 ? 8 : (p->precision == DNSG_PREC_FP32 ? 4 : 2);
+    const dnsg_dims &d = h->d;
+    h->coarse.nx = d.Xc; h->coarse.ny = d.Yc; h->coarse.nz = d.zdc;
+    h->lower.nx = d.Xf; h->lower.ny = d.Yf; h->lower.nz = d.zdf - 1;
+    h->upper = h->lower;
+    dnsg::Grid *gs[3] = {&h->coarse, &h->lower, &h->upper};
+    for (auto *g : gs)
+        for (int b = 0; b < 2; b++) {
+            const size_t bytes = g->qs() * 19 * h->elem;
+            HIPCHK(h, hipMalloc(&g->buf[b], bytes));
+            HIPCHK(h, hipMemsetAsync(g->buf[b], 0, bytes, h->stream));
+        }
+    const size_t np = (size_t)d.Xc * d.Yc * NVP;
+    for (int l = 0; l < 3; l++) {
+        HIPCHK(h, hipMalloc(&h->lev_lo[l], np * 8));
+        HIPCHK(h, hipMalloc(&h->lev_up[l], np * 8));
+        HIPCHK(h, hipMemsetAsync(h->lev_lo[l], 0, np * 8, h->stream));
+        HIPCHK(h, hipMemsetAsync(h->lev_up[l], 0, np * 8, h->stream));
+    }
+    HIPCHK(h, hipMalloc(&h->slots_lo, 4 * np * 8));
+    HIPCHK(h, hipMalloc(&h->slots_up, 4 * np * 8));
+    const std::vector<int8_t> w = wall_codes(p, &d);
+    HIPCHK(h, hipMalloc(&h->wall_lo, w.size()));
+    HIPCHK(h, hipMalloc(&h->wall_up, w.size()));
+    HIPCHK(h, hipMemcpy(h->wall_lo, w.data(), w.size(), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->wall_up, w.data(), w.size(), hipMemcpyHostToDevice));
+    h->part_cap = partial_doubles(h);
+    HIPCHK(h, hipMalloc(&h->part, h->part_cap * 8));
+    HIPCHK(h, hipMalloc(&h->d_force, 4 * 8));
+    HIPCHK(h, hipMemsetAsync(h->d_force, 0, 4 * 8, h->stream));
+    const size_t nred = 4 * ((size_t)d.zdc + 2 * (d.zdf - 1)) + 2 * 9 * (size_t)d.NZ + 64;
+    HIPCHK(h, hipMalloc(&h->d_red, nred * 8));
+    HIPCHK(h, hipMemsetAsync(h->d_red, 0, nred * 8, h->stream));
+    HIPCHK(h, hipMalloc(&h->d_acc, 14 * (size_t)d.NZ * 8));
+    HIPCHK(h, hipMemsetAsync(h->d_acc, 0, 14 * (size_t)d.NZ * 8, h->stream));
+    if ((rc = ensure_diag(h, 64))) return rc;
+    if ((rc = ensure_pinned(h, 64 + 9 * (size_t)d.NZ))) return rc;
+    h->host_wall_lo = (double *)calloc((size_t)d.Xf * d.Yf * 19, 8);
+    h->host_wall_up = (double *)calloc((size_t)d.Xf * d.Yf * 19, 8);
+    if (!h->host_wall_lo || !h->host_wall_up) {
+        h->err = "host allocation failed";
+        return DNSG_ERR_OOM;
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return DNSG_OK;
+}
+
+static int set_force(dnsg_t *h, double g) {
+    h->force = g;
+    HIPCHK(h, hipMemcpyAsync(h->d_force, &h->force, 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return DNSG_OK;
+}
+
+int dnsg_init_state(dnsg_t *h) {
+    if (!h || !h->stream) return DNSG_ERR_ARG;
+    HIPCHK(h, launch_init_lattice(h->stream, h, 0, h->coarse.c()));
+    HIPCHK(h, launch_init_lattice(h->stream, h, 1, h->lower.c()));
+    HIPCHK(h, launch_init_lattice(h->stream, h, 2, h->upper.c()));
+    const size_t np = (size_t)h->d.Xc * h->d.Yc * NVP;
+    const size_t nw = (size_t)h->d.Xf * h->d.Yf;
+    if (h->p.init_kind == DNSG_INIT_REST) {
+        // '-s' mode: walls and interface levels stay zero (calloc in main:473-883)
+        memset(h->host_wall_lo, 0, nw * 19 * 8);
+        memset(h->host_wall_up, 0, nw * 19 * 8);
+        h->wall_mass = 0.;
+        for (int l = 0; l < 3; l++) {
+            HIPCHK(h, hipMemsetAsync(h->lev_lo[l], 0, np * 8, h->stream));
+            HIPCHK(h, hipMemsetAsync(h->lev_up[l], 0, np * 8, h->stream));
+        }
+    } else {
+        // wall nodes at rest; three equal levels sampled from the initial coarse state
+        static const double wq[19] = {1. / 3., 1. / 18., 1. / 18., 1. / 18., 1. / 18., 1. / 18., 1. / 18.,
+                                      1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36.,
+                                      1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36.};
+        double m = 0.;
+        for (size_t n = 0; n < nw; n++) {
+            double r = 0.;
+            for (int a = 0; a < 19; a++) {
+                h->host_wall_lo[n * 19 + a] = wq[a];
+                h->host_wall_up[n * 19 + a] = wq[a];
+                r += wq[a];
+            }
+            m += r;
+        }
+        h->wall_mass = 2. * m;
+        HIPCHK(h, launch_sample(h->stream, h, h->coarse.c(), h->lev_lo[0], h->lev_up[0]));
+        for (int l = 1; l < 3; l++) {
+            HIPCHK(h, hipMemcpyAsync(h->lev_lo[l], h->lev_lo[0], np * 8, hipMemcpyDeviceToDevice, h->stream));
+            HIPCHK(h, hipMemcpyAsync(h->lev_up[l], h->lev_up[0], np * 8, hipMemcpyDeviceToDevice, h->stream));
+        }
+    }
+    h->ss = 0;
+    h->have_state = true;
+    return set_force(h, h->p.force_mode == DNSG_FORCE_CONST ? h->p.g_const : 0.);
+}
+
+int dnsg_upload_state(dnsg_t *h, const double *coarse, const double *lower, const double *upper, const double *lin3,
+                      const double *uin3, double force) {
+    if (!h || !h->stream) return DNSG_ERR_ARG;
+    int rc;
+    const int zdf = h->d.zdf;
+    if (coarse && (rc = put_lattice(h, coarse, h->coarse, h->d.zdc, 0))) return rc;
+    double wl = 0., wu = 0.;
+    bool wall_changed = false;
+    if (lower) {
+        if ((rc = put_lattice(h, lower, h->lower, zdf, 1))) return rc;  // stored m = k-1
+        wl = keep_wall(h, lower, 0, h->host_wall_lo);
+        wall_changed = true;
+    } else {
+        for (size_t n = 0; n < (size_t)h->d.Xf * h->d.Yf * 19; n++) wl += h->host_wall_lo[n];
+    }
+    if (upper) {
+        if ((rc = put_lattice(h, upper, h->upper, zdf, 0))) return rc;  // stored m = k
+        wu = keep_wall(h, upper, zdf - 1, h->host_wall_up);
+        wall_changed = true;
+    } else {
+        for (size_t n = 0; n < (size_t)h->d.Xf * h->d.Yf * 19; n++) wu += h->host_wall_up[n];
+    }
+    if (wall_changed) h->wall_mass = wl + wu;
+    if (lin3 && (rc = put_levels(h, lin3, h->lev_lo))) return rc;
+    if (uin3 && (rc = put_levels(h, uin3, h->lev_up))) return rc;
+    h->ss = 0;
+    h->have_state = true;
+    return set_force(h, h->p.force_mode == DNSG_FORCE_CONST ? h->p.g_const : force);
+}
+
+int dnsg_download_state(dnsg_t *h, double *coarse, double *lower, double *upper, double *lin3, double *uin3,
+                        double *force) {
+    if (!h || !h->stream) return DNSG_ERR_ARG;
+    int rc;
+    const int zdf = h->d.zdf;
+    if (coarse && (rc = get_lattice(h, coarse, h->coarse, h->d.zdc, 0))) return rc;
+    if (lower) {
+        if ((rc = get_lattice(h, lower, h->lower, zdf, 1))) return rc;
+        place_wall(h, lower, 0, h->host_wall_lo);
+    }
+    if (upper) {
+        if ((rc = get_lattice(h, upper, h->upper, zdf, 0))) return rc;
+        place_wall(h, upper, zdf - 1, h->host_wall_up);
+    }
+    if (lin3 && (rc = get_levels(h, lin3, h->lev_lo))) return rc;
+    if (uin3 && (rc = get_levels(h, uin3, h->lev_up))) return rc;
+    if (force) {
+        HIPCHK(h, hipMemcpyAsync(h->h_pinned, h->d_force, 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        *force = h->h_pinned[0];
+    }
+    return DNSG_OK;
+}
+
+int dnsg_step(dnsg_t *h, int n, dnsg_diag *diag) {
+    if (!h || !h->stream || n < 0) return DNSG_ERR_ARG;
+    if (!h->have_state) {
+        h->err = "no state: call dnsg_init_state or dnsg_upload_state first";
+        return DNSG_ERR_STATE;
+    }
+    int rc;
+    const int chunk = 256;
+    for (int s0 = 0; s0 < n; s0 += chunk) {
+        const int m = (n - s0 < chunk) ? n - s0 : chunk;
+        if ((rc = ensure_diag(h, m))) return rc;
+        const long ss0 = h->ss;
+        for (int s = 0; s < m; s++)
+            if ((rc = do_step(h, h->d_diag + (size_t)s * DIAG_WORDS))) return rc;
+        if ((rc = ensure_pinned(h, (size_t)m * DIAG_WORDS))) return rc;
+        HIPCHK(h, hipMemcpyAsync(h->h_pinned, h->d_diag, sizeof(double) * DIAG_WORDS * m, hipMemcpyDeviceToHost,
+                                 h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        for (int s = 0; s < m; s++) {
+            const double *rec = h->h_pinned + (size_t)s * DIAG_WORDS;
+            if (diag) fill_diag(h, rec, ss0 + s, diag + s0 + s);
+            if (isnan(rec[7])) {
+                h->err = "NaN in wall shear stress";
+                return DNSG_ERR_NAN;
+            }
+        }
+        h->force = h->h_pinned[(size_t)(m - 1) * DIAG_WORDS + 1];
+    }
+    return DNSG_OK;
+}
+
+int dnsg_run_timed(dnsg_t *h, int n, dnsg_diag *diag, double *elapsed_ms) {
+    if (!h || !h->stream || n < 1) return DNSG_ERR_ARG;
+    if (!h->have_state) {
+        h->err = "no state: call dnsg_init_state or dnsg_upload_state first";
+        return DNSG_ERR_STATE;
+    }
+    int rc;
+    if ((rc = ensure_diag(h, n))) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(h, hipEventCreate(&e0));
+    HIPCHK(h, hipEventCreate(&e1));
+    const long ss0 = h->ss;
+    HIPCHK(h, hipEventRecord(e0, h->stream));
+    for (int s = 0; s < n; s++)
+        if ((rc = do_step(h, h->d_diag + (size_t)s * DIAG_WORDS))) return rc;
+    HIPCHK(h, hipEventRecord(e1, h->stream));
+    HIPCHK(h, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (elapsed_ms) *elapsed_ms = ms;
+    if ((rc = ensure_pinned(h, (size_t)n * DIAG_WORDS))) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->h_pinned, h->d_diag, sizeof(double) * DIAG_WORDS * n, hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int s = 0; s < n; s++) {
+        const double *rec = h->h_pinned + (size_t)s * DIAG_WORDS;
+        if (diag) fill_diag(h, rec, ss0 + s, diag + s);
+        if (isnan(rec[7])) {
+            h->err = "NaN in wall shear stress";
+            return DNSG_ERR_NAN;
+        }
+    }
+    return DNSG_OK;
+}
+
+int dnsg_kernel_times(dnsg_t *h, int n, double *ms) {
+    if (!h || !h->stream || n < 1 || !ms) return DNSG_ERR_ARG;
+    if (!h->have_state) {
+        h->err = "no state: call dnsg_init_state or dnsg_upload_state first";
+        return DNSG_ERR_STATE;
+    }
+    int rc;
+    if ((rc = ensure_diag(h, 1))) return rc;
+    double acc[SEC_N] = {0., 0., 0., 0., 0.};
+    int cnt[SEC_N] = {0, 0, 0, 0, 0};
+    for (int s = 0; s < n; s++) {
+        SecTimer T;
+        rc = do_step(h, h->d_diag, &T);
+        if (!rc) {
+            hipError_t e = hipEventSynchronize(T.ev.back());
+            for (size_t i = 1; i < T.ev.size() && e == hipSuccess; i++) {
+                float t = 0.f;
+                e = hipEventElapsedTime(&t, T.ev[i - 1], T.ev[i]);
+                acc[T.tag[i]] += t;
+                cnt[T.tag[i]]++;
+            }
+            if (e != hipSuccess) {
+                h->err = std::string("event timing: ") + hipGetErrorString(e);
+                rc = DNSG_ERR_HIP;
+            }
+        }
+        for (hipEvent_t e : T.ev) (void)hipEventDestroy(e);
+        if (rc) return rc;
+    }
+    // average time per launch (group) of each section
+    for (int k = 0; k < SEC_N; k++) ms[k] = cnt[k] ? acc[k] / cnt[k] : 0.;
+    return DNSG_OK;
+}
+
+// ---- reference-granular entry points ------------------------------------------------
+
+static int scalar_call(dnsg_t *h, long ss, dnsg_diag *d) {
+    HIPCHK(h, hipMemcpyAsync(h->h_pinned, h->d_diag, sizeof(double) * DIAG_WORDS, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (d) {
+        dnsg_diag t;
+        fill_diag(h, h->h_pinned, ss, &t);
+        d->time = t.time;
+        return 1;
+    }
+    return 0;
+}
+
+int dnsg_encalc2(dnsg_t *h, long ss, dnsg_diag *d) {
+    if (!h || !h->stream) return DNSG_ERR_ARG;
+    HIPCHK(h, launch_encalc2(h->stream, h, h->d_diag));
+    scalar_call(h, ss, d);
+    const double *rec = h->h_pinned;
+    h->force = rec[1];
+    if (d) { d->force = rec[1]; d->momentum = rec[2]; d->density = rec[3]; d->flowrate = rec[4]; }
+    return DNSG_OK;
+}
+
+int dnsg_wshearstr(dnsg_t *h, long ss, dnsg_diag *d) {
+    if (!h || !h->stream) return DNSG_ERR_ARG;
+    HIPCHK(h, launch_wshear(h->stream, h, h->d_diag));
+    scalar_call(h, ss, d);
+    const double *rec = h->h_pinned;
+    if (d) { d->str_up = rec[5]; d->str_down = rec[6]; d->stress = rec[7]; d->pgrad = rec[8]; }
+    if (isnan(rec[7])) {
+        h->err = "NaN in wall shear stress";
+        return DNSG_ERR_NAN;
+    }
+    return DNSG_OK;
+}
+
+int dnsg_xcomputations(dnsg_t *h) { return h ? do_xcomputations(h) : DNSG_ERR_ARG; }
+int dnsg_xfcomputations(dnsg_t *h) { return h ? do_xfcomputations(h) : DNSG_ERR_ARG; }
+
+int dnsg_ctfdtransfer(dnsg_t *h, int iter) {
+    if (!h || iter < 1 || iter > h->d.GR) return DNSG_ERR_ARG;
+    HIPCHK(h, launch_interface(h->stream, h, iter));
+    return DNSG_OK;
+}
+
+int dnsg_ftcdtransfer(dnsg_t *h) {
+    if (!h) return DNSG_ERR_ARG;
+    HIPCHK(h, launch_ftcd(h->stream, h));
+    return DNSG_OK;
+}
+
+int dnsg_exchange(dnsg_t *h) {
+    if (!h) return DNSG_ERR_ARG;
+    h->coarse.swap();
+    h->ss++;
+    return DNSG_OK;
+}
+
+int dnsg_fexchange(dnsg_t *h) {
+    if (!h) return DNSG_ERR_ARG;
+    h->lower.swap();
+    h->upper.swap();
+    return DNSG_OK;
+}
+
+// ---- reductions and snapshots ----------------------------------------------------------
+
+int dnsg_mass(dnsg_t *h, double *avg) {
+    if (!h || !avg) return DNSG_ERR_ARG;
+    HIPCHK(h, launch_mass(h->stream, h, h->d_red));
+    HIPCHK(h, hipMemcpyAsync(h->h_pinned, h->d_red, 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    *avg = h->h_pinned[0];
+    return DNSG_OK;
+}
+
+static double *stats_scratch(dnsg_t *h) { return h->d_red + 4 * ((size_t)h->d.zdc + 2 * (h->d.zdf - 1)); }
+
+int dnsg_statistics(dnsg_t *h, double *out) {
+    if (!h || !out) return DNSG_ERR_ARG;
+    double *dst = stats_scratch(h);
+    const size_t n = 9 * (size_t)h->d.NZ;
+    HIPCHK(h, hipMemsetAsync(dst, 0, n * 8, h->stream));
+    HIPCHK(h, launch_statistics(h->stream, h, dst));
+    HIPCHK(h, hipMemcpyAsync(h->h_pinned, dst, n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    memcpy(out, h->h_pinned, n * 8);
+    return DNSG_OK;
+}
+
+int dnsg_stats_accumulate(dnsg_t *h) {
+    if (!h) return DNSG_ERR_ARG;
+    double *dst = stats_scratch(h);
+    const int NZ = h->d.NZ;
+    HIPCHK(h, hipMemsetAsync(dst, 0, 9 * (size_t)NZ * 8, h->stream));
+    HIPCHK(h, launch_statistics(h->stream, h, dst));
+    HIPCHK(h, launch_fold_stats(h->stream, h->d_acc, dst, NZ));
+    h->acc_count++;
+    return DNSG_OK;
+}
+
+int dnsg_stats_average(dnsg_t *h, double *out, long *count) {
+    if (!h || !out) return DNSG_ERR_ARG;
+    const int NZ = h->d.NZ;
+    const size_t n = 14 * (size_t)NZ;
+    if (int rc = ensure_pinned(h, n)) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->h_pinned, h->d_acc, n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const double T = h->acc_count > 0 ? (double)h->acc_count : 1.;
+    const double *a = h->h_pinned;
+    for (int k = 0; k < NZ; k++) {
+        auto m = [&](int r) { return a[(size_t)r * NZ + k] / T; };
+        const double U = m(0), V = m(1), W = m(2), D = m(7);
+        auto root = [](double v) { return v > 0. ? sqrt(v) : 0.; };
+        out[0 * NZ + k] = U;
+        out[1 * NZ + k] = V;
+        out[2 * NZ + k] = W;
+        out[3 * NZ + k] = root(m(3) + (m(9) - U * U));
+        out[4 * NZ + k] = root(m(4) + (m(10) - V * V));
+        out[5 * NZ + k] = root(m(5) + (m(11) - W * W));
+        out[6 * NZ + k] = m(6) + (m(12) - U * W);
+        out[7 * NZ + k] = 1. + D;
+        out[8 * NZ + k] = root(m(8) + (m(13) - D * D));
+    }
+    if (count) *count = h->acc_count;
+    return DNSG_OK;
+}
+
+int dnsg_stats_reset(dnsg_t *h) {
+    if (!h) return DNSG_ERR_ARG;
+    HIPCHK(h, hipMemsetAsync(h->d_acc, 0, 14 * (size_t)h->d.NZ * 8, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->acc_count = 0;
+    return DNSG_OK;
+}
+
+int dnsg_velocity_snapshot(dnsg_t *h, int grid, double *vel, double *den) {
+    if (!h || grid < 0 || grid > 2) return DNSG_ERR_ARG;
+    const int X = grid == 0 ? h->d.Xc : h->d.Xf, Y = grid == 0 ? h->d.Yc : h->d.Yf;
+    const int Z = grid == 0 ? h->d.zdc : h->d.zdf;
+    const size_t nn = (size_t)X * Y * Z;
+    std::vector<double> aos(nn * 19);
+    int rc;
+    if (grid == 0) rc = dnsg_download_state(h, aos.data(), nullptr, nullptr, nullptr, nullptr, nullptr);
+    else if (grid == 1) rc = dnsg_download_state(h, nullptr, aos.data(), nullptr, nullptr, nullptr, nullptr);
+    else rc = dnsg_download_state(h, nullptr, nullptr, aos.data(), nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    static const double E[3][19] = {{0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, 1, -1, -1, 0, 0, 0, 0},
+                                    {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, 1, -1, -1},
+                                    {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1}};
+    for (size_t m = 0; m < nn; m++) {  // vel_print.c:86-117
+        const double *f = aos.data() + m * 19;
+        double ux = 0., uy = 0., uz = 0., dens = 0.;
+        for (int a = 0; a < 19; a++) {
+            ux += E[0][a] * f[a];
+            uy += E[1][a] * f[a];
+            uz += E[2][a] * f[a];
+            dens += f[a];
+        }
+        if (dens > 0.) { ux /= dens; uy /= dens; uz /= dens; }
+        else { ux = 0.; uy = 0.; uz = 0.; dens = 0.; }
+        if (vel) { vel[3 * m] = ux; vel[3 * m + 1] = uy; vel[3 * m + 2] = uz; }
+        if (den) den[m] = dens;
+    }
+    return DNSG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,115 @@
+// internal.hpp -- device-side state of one solver handle and the kernel launchers.
+//
+// Layout in HBM (structure of arrays, x fastest, one plane per direction):
+//   lattice g, direction q, stored layer z, row y, column x ->
+//       base[g] + q*qs + (z*ny + y)*nx + x,   qs = nx*ny*nz
+// The coarse grid stores all zdc layers.  A band stores its zds fluid layers
+// only (the wall layer is never collided or streamed and is kept on the host):
+//   lower band: stored layer m <-> reference k = m+1  (wall below m = 0,
+//               interface layer m = zds-1)
+//   upper band: stored layer m <-> reference k = m    (interface layer m = 0,
+//               wall above m = zds-1)
+// Interface planes are [23][Yc][Xc]; the interpolation slots are
+// [4][23][Yc][Xc] (value, d/dx, d/dy, d2/dxdy).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/dnsg.h"
+
+namespace dnsg {
+
+enum BoundKind { BOUND_DROP = 0, BOUND_WALL = 1 };
+
+// one lattice as the streaming kernels see it
+struct LatArgs {
+    const void *cur;   // pre-collision state (read)
+    void *nxt;         // post-stream state (written)
+    int nx, ny, nz;
+    int lo_kind, hi_kind;  // below z = 0 / above z = nz-1
+    const int8_t *wlo;     // wall codes [ny][nx] (-1 solid, -2 gas) of the wall below
+    const int8_t *whi;     // ... and above
+    double gscale;         // G = gscale * force (1 for coarse, 1/GR for the bands)
+    double omega;          // 1/tau
+};
+
+struct Grid {
+    void *buf[2] = {nullptr, nullptr};  // A/B lattices
+    int cur = 0;                        // index of the current (pre-collision) lattice
+    int nx = 0, ny = 0, nz = 0;
+    size_t qs() const { return (size_t)nx * ny * nz; }
+    size_t nodes() const { return qs(); }
+    void *c() const { return buf[cur]; }
+    void *n() const { return buf[cur ^ 1]; }
+    void swap() { cur ^= 1; }
+};
+
+// number of doubles in one diag record written by the reduction finalisers
+constexpr int DIAG_WORDS = 9;
+// per-row partial sums kept for one reduction pass
+constexpr int MAX_PARTIAL_Q = 5;
+
+}  // namespace dnsg
+
+struct dnsg_ctx {
+    dnsg_params p;
+    dnsg_dims d;
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t elem = 8;  // bytes per stored distribution value
+
+    dnsg::Grid coarse, lower, upper;
+    // interface time levels, newest first; rotated by pointer swaps
+    double *lev_lo[3] = {nullptr, nullptr, nullptr};
+    double *lev_up[3] = {nullptr, nullptr, nullptr};
+    double *slots_lo = nullptr, *slots_up = nullptr;  // [4][23][Yc][Xc]
+    int8_t *wall_lo = nullptr, *wall_up = nullptr;     // [Yf][Xf]
+
+    // reductions
+    double *part = nullptr;     // row partials
+    size_t part_cap = 0;        // doubles
+    double *d_force = nullptr;  // device copy of V.force[0]
+    double *d_diag = nullptr;   // DIAG_WORDS per step
+    int diag_cap = 0;
+    double *d_red = nullptr;    // small scratch for finalised vectors (statistics)
+    double *h_pinned = nullptr; // pinned host staging for scalars
+    size_t h_pinned_cap = 0;
+
+    // running statistics accumulators [14][NZ] + count
+    double *d_acc = nullptr;
+    long acc_count = 0;
+
+    // host copies of the band wall layers (never touched by the step) and their
+    // mass contribution, for checkpoints and the mass check
+    double *host_wall_lo = nullptr, *host_wall_up = nullptr;  // [Xf][Yf][19]
+    double wall_mass = 0.;
+
+    double force = 0.;  // host mirror of V.force[0]
+    long ss = 0;        // next step index of the time loop
+    bool have_state = false;
+};
+
+namespace dnsg {
+
+// launchers (stream-ordered, no host synchronisation)
+hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, const double *force, int prec);
+hipError_t launch_init_lattice(hipStream_t s, dnsg_ctx *h, int grid, void *dst);
+hipError_t launch_aos_to_soa(hipStream_t s, const double *aos, void *soa, int nx, int ny, int nz_aos, int z0,
+                             int nz, int prec);
+hipError_t launch_soa_to_aos(hipStream_t s, const void *soa, double *aos, int nx, int ny, int nz_aos, int z0,
+                             int nz, int prec);
+hipError_t launch_sample(hipStream_t s, dnsg_ctx *h, const void *coarse_state, double *lo, double *up);
+hipError_t launch_interface(hipStream_t s, dnsg_ctx *h, int iter);
+hipError_t launch_ftcd(hipStream_t s, dnsg_ctx *h);
+hipError_t launch_encalc2(hipStream_t s, dnsg_ctx *h, double *diag_rec);
+hipError_t launch_wshear(hipStream_t s, dnsg_ctx *h, double *diag_rec);
+hipError_t launch_mass(hipStream_t s, dnsg_ctx *h, double *out);
+hipError_t launch_statistics(hipStream_t s, dnsg_ctx *h, double *out9nz);
+hipError_t launch_fold_stats(hipStream_t s, double *acc, const double *st, int NZ);
+size_t partial_doubles(const dnsg_ctx *h);
+
+}  // namespace dnsg

@@ -1,0 +1,429 @@
+// k_reduce.hip -- FP64 reductions of the time step: bulk flux and density with
+// the flux controller (encalc2.c:17-147), wall shear stress (wshearstr.c:15-169),
+// the global mass check (main:1193-1250) and the plane statistics
+// (statistics.c:15-300).
+//
+// Every reduction is two-level and deterministic: one workgroup per lattice row
+// sums its row (wave64 shuffles, then LDS across the 4 waves) into a row
+// partial; a single-workgroup finaliser sums the row partials in a fixed order,
+// applies the reference's weights and normalisations and writes the result to
+// device memory (the force the next collision reads, the diag record).  No
+// atomics: the result depends only on the lattice contents, never on timing or
+// on other work in flight.
+#include <algorithm>
+
+#include "internal.hpp"
+#include "lattice.hpp"
+
+namespace dnsg {
+
+constexpr int RB = 256;  // threads per reduction workgroup
+constexpr int NMOM = 5;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// block-wide sum of K values; result valid in thread 0
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double (*lds)[RB / 64]) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const double s = wave_sum(v[k]);
+        if (lane == 0) lds[k][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double s = 0.;
+            for (int w = 0; w < RB / 64; w++) s += lds[k][w];
+            v[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+template <class S>
+__device__ __forceinline__ void node_moments(const S *lat, size_t qs, size_t idx, double &dens, double &ux, double &uy,
+                                             double &uz) {
+    double f[Q];
+    unroll<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (sizeof(S) == 8) f[q] = lat[q * qs + idx];
+        else f[q] = (double)load_s(lat + q * qs + idx);
+    });
+    ux = first_moment<0>(f);
+    uy = first_moment<1>(f);
+    uz = first_moment<2>(f);
+    dens = zeroth_moment(f);
+    if constexpr (sizeof(S) != 8) dens = 1. + dens;
+}
+
+// one lattice for the row kernels
+struct RedLat {
+    const void *lat;
+    int nx, ny, nz;
+    double *part;          // NMOM doubles per row
+    const double *means;   // pass 2: [nz][4] plane means (U, V, W, rho) or null
+};
+struct RedSet {
+    RedLat g[3];
+};
+
+// MODE 1: sums of rho, rho u_x, u_x, u_y, u_z (u = rho u / rho)
+// MODE 2: sums of u'^2, v'^2, w'^2, u'w', rho'^2 about the plane means
+// MODE 3: sums of u^2, v^2, w^2, u w, rho^2 (raw second moments)
+template <class S, int MODE>
+__global__ __launch_bounds__(RB) void k_row_moments(RedSet R) {
+    const RedLat &L = R.g[blockIdx.y];
+    const int row = blockIdx.x;
+    if (row >= L.ny * L.nz) return;
+    __shared__ double lds[NMOM][RB / 64];
+    const int z = row / L.ny;
+    const size_t qs = (size_t)L.nx * L.ny * L.nz, row0 = (size_t)row * L.nx;
+    const S *lat = static_cast<const S *>(L.lat);
+    double acc[NMOM] = {0., 0., 0., 0., 0.};
+    double U = 0., V = 0., Wm = 0., D = 0.;
+    if (MODE == 2) {
+        U = L.means[z * 4 + 0]; V = L.means[z * 4 + 1]; Wm = L.means[z * 4 + 2]; D = L.means[z * 4 + 3];
+    }
+    for (int x = threadIdx.x; x < L.nx; x += RB) {
+        double dens, ux, uy, uz;
+        node_moments(lat, qs, row0 + x, dens, ux, uy, uz);
+        if (MODE == 1) {
+            acc[0] += dens; acc[1] += ux; acc[2] += ux / dens; acc[3] += uy / dens; acc[4] += uz / dens;
+        } else if (MODE == 2) {
+            const double uf = (ux / dens) - U, vf = (uy / dens) - V, wf = (uz / dens) - Wm, df = dens - D;
+            acc[0] += uf * uf; acc[1] += vf * vf; acc[2] += wf * wf; acc[3] += uf * wf; acc[4] += df * df;
+        } else {
+            const double u = ux / dens, v = uy / dens, w = uz / dens;
+            acc[0] += u * u; acc[1] += v * v; acc[2] += w * w; acc[3] += u * w; acc[4] += dens * dens;
+        }
+    }
+    block_sum<NMOM>(acc, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NMOM; k++) L.part[(size_t)row * NMOM + k] = acc[k];
+}
+
+template <int MODE>
+static void row_moments(hipStream_t s, int prec, const RedSet &R, int nlat) {
+    int rows = 0;
+    for (int i = 0; i < nlat; i++) rows = std::max(rows, R.g[i].ny * R.g[i].nz);
+    dim3 grd(rows, nlat), blk(RB);
+    if (prec == DNSG_PREC_FP64) hipLaunchKernelGGL((k_row_moments<double, MODE>), grd, blk, 0, s, R);
+    else if (prec == DNSG_PREC_FP32) hipLaunchKernelGGL((k_row_moments<float, MODE>), grd, blk, 0, s, R);
+    else hipLaunchKernelGGL((k_row_moments<__half, MODE>), grd, blk, 0, s, R);
+}
+
+// partial-buffer layout: [coarse rows | lower rows | upper rows] x NMOM, then the
+// wall-stress partials
+struct PartLayout {
+    size_t off_c, off_l, off_u, off_ws, total;
+};
+static PartLayout part_layout(const dnsg_ctx *h) {
+    PartLayout P;
+    const size_t rc = (size_t)h->coarse.ny * h->coarse.nz, rl = (size_t)h->lower.ny * h->lower.nz;
+    P.off_c = 0;
+    P.off_l = P.off_c + rc * NMOM;
+    P.off_u = P.off_l + rl * NMOM;
+    P.off_ws = P.off_u + rl * NMOM;
+    P.total = P.off_ws + (size_t)h->lower.ny * 4;
+    return P;
+}
+
+static RedSet make_set(dnsg_ctx *h, const double *means_c, const double *means_l, const double *means_u) {
+    PartLayout P = part_layout(h);
+    RedSet R;
+    R.g[0] = {h->coarse.c(), h->coarse.nx, h->coarse.ny, h->coarse.nz, h->part + P.off_c, means_c};
+    R.g[1] = {h->lower.c(), h->lower.nx, h->lower.ny, h->lower.nz, h->part + P.off_l, means_l};
+    R.g[2] = {h->upper.c(), h->upper.nx, h->upper.ny, h->upper.nz, h->part + P.off_u, means_u};
+    return R;
+}
+
+// ---- encalc2 ---------------------------------------------------------------------------
+struct EncArgs {
+    const double *pc, *pl, *pu;
+    int Yc, zdc, Yf, zds, GR, zdf, NZs, Xf, force_mode;
+    double ub, g_const;
+    double *force, *diag;
+};
+
+__global__ __launch_bounds__(RB) void k_fin_encalc2(EncArgs A) {
+    __shared__ double lds[4][RB / 64];
+    double v[4] = {0., 0., 0., 0.};  // momentum, denst, cmomentum, cdenst
+    const double g3 = 1. * A.GR * A.GR * A.GR, g3h = 0.5 * A.GR * A.GR * A.GR;
+    // lower band k = 1 .. zdf-GR-1  <->  stored m = 0 .. zdf-GR-2, weight 1
+    const int nl = A.Yf * (A.zdf - A.GR - 1);
+    for (int r = threadIdx.x; r < nl; r += RB) { v[0] += A.pl[r * NMOM + 1]; v[1] += A.pl[r * NMOM + 0]; }
+    // coarse: every layer, GR^3 (half at k = 0, zdc-1)
+    const int nc = A.Yc * A.zdc;
+    for (int r = threadIdx.x; r < nc; r += RB) {
+        const int z = r / A.Yc;
+        const double c = (z == 0 || z == A.zdc - 1) ? g3h : g3;
+        v[0] += c * A.pc[r * NMOM + 1]; v[1] += c * A.pc[r * NMOM + 0];
+        v[2] += c * A.pc[r * NMOM + 1]; v[3] += c * A.pc[r * NMOM + 0];
+    }
+    // upper band k = GR .. zdf-2  <->  stored m = GR .. zds-1, half at k = GR
+    const int nu0 = A.Yf * A.GR, nu1 = A.Yf * A.zds;
+    for (int r = nu0 + threadIdx.x; r < nu1; r += RB) {
+        const double c = (r / A.Yf == A.GR) ? 0.5 : 1.;
+        v[0] += c * A.pu[r * NMOM + 1]; v[1] += c * A.pu[r * NMOM + 0];
+    }
+    block_sum<4>(v, lds);
+    if (threadIdx.x == 0) {
+        const double vol = ((double)((A.Xf + 1 - 1.) * (A.Yf + 1 - 1.) * (A.NZs - 2.)));  // encalc2.c:109
+        const double momentum = v[0] / vol, denst = v[1] / vol;
+        const double G = (A.force_mode == DNSG_FORCE_FLUX) ? A.ub - momentum : A.g_const;  // :129
+        A.force[0] = G;
+        A.force[1] = v[2] / vol;
+        A.force[2] = v[3] / vol;
+        if (A.diag) {
+            A.diag[1] = G;
+            A.diag[2] = momentum;
+            A.diag[3] = denst;
+            A.diag[4] = momentum * ((A.Yf) * (A.NZs - 2)) / denst;  // :138
+        }
+    }
+}
+
+hipError_t launch_encalc2(hipStream_t s, dnsg_ctx *h, double *diag_rec) {
+    RedSet R = make_set(h, nullptr, nullptr, nullptr);
+    row_moments<1>(s, h->p.precision, R, 3);
+    EncArgs A;
+    A.pc = R.g[0].part; A.pl = R.g[1].part; A.pu = R.g[2].part;
+    A.Yc = h->d.Yc; A.zdc = h->d.zdc; A.Yf = h->d.Yf; A.zds = h->lower.nz; A.GR = h->d.GR; A.zdf = h->d.zdf;
+    A.NZs = h->d.NZs; A.Xf = h->d.Xf; A.force_mode = h->p.force_mode; A.ub = h->p.ub; A.g_const = h->p.g_const;
+    A.force = h->d_force; A.diag = diag_rec;
+    hipLaunchKernelGGL(k_fin_encalc2, dim3(1), dim3(RB), 0, s, A);
+    return hipGetLastError();
+}
+
+// ---- wall shear stress ---------------------------------------------------------------------
+// per (x, y): lower wall  -2u(k=1) + 3u(k=2) - u(k=3), rho(k=1)
+//             upper wall  -(2u(zdf-2) - 3u(zdf-3) + u(zdf-4)), rho(zdf-4)
+template <class S>
+__global__ __launch_bounds__(RB) void k_wshear_rows(const void *lo, const void *up, int nx, int ny, int nz,
+                                                    double *part) {
+    __shared__ double lds[4][RB / 64];
+    const int y = blockIdx.x;
+    const size_t qs = (size_t)nx * ny * nz;
+    const S *L = static_cast<const S *>(lo), *U = static_cast<const S *>(up);
+    double acc[4] = {0., 0., 0., 0.};
+    for (int x = threadIdx.x; x < nx; x += RB) {
+        double d1, d2, d3, a1, a2, a3, t0, t1;
+        node_moments(L, qs, ((size_t)0 * ny + y) * nx + x, d1, a1, t0, t1);
+        node_moments(L, qs, ((size_t)1 * ny + y) * nx + x, d2, a2, t0, t1);
+        node_moments(L, qs, ((size_t)2 * ny + y) * nx + x, d3, a3, t0, t1);
+        const double ux1 = a1 / d1, ux2 = a2 / d2, ux3 = a3 / d3;
+        acc[0] += (-2. * ux1 + 3. * ux2 - ux3);
+        acc[1] += d1;
+        node_moments(U, qs, ((size_t)(nz - 1) * ny + y) * nx + x, d1, a1, t0, t1);
+        node_moments(U, qs, ((size_t)(nz - 2) * ny + y) * nx + x, d2, a2, t0, t1);
+        node_moments(U, qs, ((size_t)(nz - 3) * ny + y) * nx + x, d3, a3, t0, t1);
+        const double vx1 = a1 / d1, vx2 = a2 / d2, vx3 = a3 / d3;
+        acc[2] += -(2. * vx1 - 3. * vx2 + vx3);
+        acc[3] += d3;
+    }
+    block_sum<4>(acc, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; k++) part[(size_t)y * 4 + k] = acc[k];
+}
+
+struct WsArgs {
+    const double *part;
+    int Xf, Yf, NZs, GR;
+    double omega_f;
+    const double *force;
+    double *diag;
+    double *out;  // [4]: str_up, str_down, stress, pgrad
+};
+
+__global__ __launch_bounds__(RB) void k_fin_wshear(WsArgs A) {
+    __shared__ double lds[4][RB / 64];
+    double v[4] = {0., 0., 0., 0.};
+    for (int r = threadIdx.x; r < A.Yf; r += RB)
+        for (int k = 0; k < 4; k++) v[k] += A.part[(size_t)r * 4 + k];
+    block_sum<4>(v, lds);
+    if (threadIdx.x == 0) {
+        const double mu = (1. / 6.) * (2. / A.omega_f - 1);              // wshearstr.c:181
+        const double area = ((double)((A.Xf + 1 - 1.) * (A.Yf + 1 - 1.)));
+        double davg = v[1] / area, stdown = v[0] / area;                   // :218-219
+        stdown *= mu * davg;                                               // :227
+        double davgu = v[3] / area, stup = v[2] / area;                    // :266-267
+        double pgrad = A.force[0];
+        pgrad = pgrad * (A.NZs - 2.) / (2. * A.GR);                        // :278
+        stup *= mu * davgu;                                                // :280
+        const double stress = 0.5 * (stup + stdown);                       // :284
+        if (A.diag) { A.diag[5] = stup; A.diag[6] = stdown; A.diag[7] = stress; A.diag[8] = pgrad; }
+        if (A.out) { A.out[0] = stup; A.out[1] = stdown; A.out[2] = stress; A.out[3] = pgrad; }
+    }
+}
+
+hipError_t launch_wshear(hipStream_t s, dnsg_ctx *h, double *diag_rec) {
+    PartLayout P = part_layout(h);
+    double *part = h->part + P.off_ws;
+    const Grid &l = h->lower;
+    dim3 grd(l.ny), blk(RB);
+    if (h->p.precision == DNSG_PREC_FP64)
+        hipLaunchKernelGGL(k_wshear_rows<double>, grd, blk, 0, s, h->lower.c(), h->upper.c(), l.nx, l.ny, l.nz, part);
+    else if (h->p.precision == DNSG_PREC_FP32)
+        hipLaunchKernelGGL(k_wshear_rows<float>, grd, blk, 0, s, h->lower.c(), h->upper.c(), l.nx, l.ny, l.nz, part);
+    else
+        hipLaunchKernelGGL(k_wshear_rows<__half>, grd, blk, 0, s, h->lower.c(), h->upper.c(), l.nx, l.ny, l.nz, part);
+    WsArgs A;
+    A.part = part; A.Xf = h->d.Xf; A.Yf = h->d.Yf; A.NZs = h->d.NZs; A.GR = h->d.GR; A.omega_f = h->d.omega_f;
+    A.force = h->d_force; A.diag = diag_rec; A.out = nullptr;
+    hipLaunchKernelGGL(k_fin_wshear, dim3(1), dim3(RB), 0, s, A);
+    return hipGetLastError();
+}
+
+// ---- mass check ---------------------------------------------------------------------------
+struct MassArgs {
+    const double *pc, *pl, *pu;
+    int Yc, zdc, Yf, zds, GR, Xf, NZs;
+    double wall_mass;
+    double *out;
+};
+
+__global__ __launch_bounds__(RB) void k_fin_mass(MassArgs A) {
+    __shared__ double lds[1][RB / 64];
+    double v[1] = {0.};
+    const double g3 = 1. * A.GR * A.GR * A.GR, g3h = 0.5 * A.GR * A.GR * A.GR;
+    const int nb = A.Yf * A.zds;
+    // lower band k = 1 .. zdf-1 (stored m = 0 .. zds-1), half at the interface k = zdf-1
+    for (int r = threadIdx.x; r < nb; r += RB) v[0] += ((r / A.Yf == A.zds - 1) ? 0.5 : 1.) * A.pl[r * NMOM];
+    // coarse k = 1 .. zdc-2, GR^3 (half at k = 1, zdc-2)
+    for (int r = A.Yc + threadIdx.x; r < A.Yc * (A.zdc - 1); r += RB) {
+        const int z = r / A.Yc;
+        v[0] += ((z == 1 || z == A.zdc - 2) ? g3h : g3) * A.pc[r * NMOM];
+    }
+    // upper band k = 0 .. zdf-2, half at the interface k = 0
+    for (int r = threadIdx.x; r < nb; r += RB) v[0] += ((r / A.Yf == 0) ? 0.5 : 1.) * A.pu[r * NMOM];
+    block_sum<1>(v, lds);
+    if (threadIdx.x == 0)
+        A.out[0] = (v[0] + A.wall_mass) / ((double)((A.Xf + 1 - 1.) * (A.Yf + 1 - 1.) * (A.NZs - 2.)));
+}
+
+hipError_t launch_mass(hipStream_t s, dnsg_ctx *h, double *out) {
+    RedSet R = make_set(h, nullptr, nullptr, nullptr);
+    row_moments<1>(s, h->p.precision, R, 3);
+    MassArgs A;
+    A.pc = R.g[0].part; A.pl = R.g[1].part; A.pu = R.g[2].part;
+    A.Yc = h->d.Yc; A.zdc = h->d.zdc; A.Yf = h->d.Yf; A.zds = h->lower.nz; A.GR = h->d.GR; A.Xf = h->d.Xf;
+    A.NZs = h->d.NZs; A.wall_mass = h->wall_mass; A.out = out;
+    hipLaunchKernelGGL(k_fin_mass, dim3(1), dim3(RB), 0, s, A);
+    return hipGetLastError();
+}
+
+// ---- plane statistics ------------------------------------------------------------------------
+// profile slots: lower k = 1..zdf-1 -> k; coarse k = 2..zdc-3 -> zdf-2+k;
+// upper k = 0..zdf-2 -> zdf+zdc-4+k (statistics.c:63, :97, :131)
+struct PlaneArgs {
+    const double *pc, *pl, *pu;
+    int Yc, Yf, zdc, zdf, NZ;
+    double nc, nf;      // nodes per plane (Xc*Yc, Xf*Yf)
+    int pass;           // 1: means, 2: fluctuations, 3: raw second moments
+    double *stats;      // [9][NZ] in the turb-fld order
+    double *means_c, *means_l, *means_u;  // [nz][4]
+};
+
+__global__ __launch_bounds__(RB) void k_fin_planes(PlaneArgs A) {
+    __shared__ double lds[NMOM][RB / 64];
+    const int slot = blockIdx.x;
+    const double *part = nullptr;
+    int ny = 0, z = -1;
+    double nn = 1.;
+    double *mean_out = nullptr;
+    if (slot >= 1 && slot <= A.zdf - 1) { part = A.pl; ny = A.Yf; z = slot - 1; nn = A.nf; mean_out = A.means_l; }
+    else if (slot >= A.zdf && slot <= A.zdf + A.zdc - 5) {
+        part = A.pc; ny = A.Yc; z = slot - (A.zdf - 2); nn = A.nc; mean_out = A.means_c;
+    } else if (slot >= A.zdf + A.zdc - 4 && slot <= 2 * A.zdf + A.zdc - 6) {
+        part = A.pu; ny = A.Yf; z = slot - (A.zdf + A.zdc - 4); nn = A.nf; mean_out = A.means_u;
+    }
+    double v[NMOM] = {0., 0., 0., 0., 0.};
+    if (part)
+        for (int y = threadIdx.x; y < ny; y += RB)
+            for (int k = 0; k < NMOM; k++) v[k] += part[((size_t)z * ny + y) * NMOM + k];
+    block_sum<NMOM>(v, lds);
+    if (threadIdx.x != 0) return;
+    double *st = A.stats;
+    const int NZ = A.NZ;
+    if (A.pass == 1) {
+        // v = (rho, rho u_x, u, v, w): means U, V, W, rho
+        const double U = part ? v[2] / nn : 0., V = part ? v[3] / nn : 0., W = part ? v[4] / nn : 0.,
+                     D = part ? v[0] / nn : 0.;
+        st[0 * NZ + slot] = U; st[1 * NZ + slot] = V; st[2 * NZ + slot] = W; st[7 * NZ + slot] = D;
+        if (mean_out) { mean_out[z * 4 + 0] = U; mean_out[z * 4 + 1] = V; mean_out[z * 4 + 2] = W; mean_out[z * 4 + 3] = D; }
+    } else if (A.pass == 2) {
+        st[3 * NZ + slot] = sqrt(part ? v[0] / nn : 0.);
+        st[4 * NZ + slot] = sqrt(part ? v[1] / nn : 0.);
+        st[5 * NZ + slot] = sqrt(part ? v[2] / nn : 0.);
+        st[6 * NZ + slot] = part ? v[3] / nn : 0.;
+        st[8 * NZ + slot] = sqrt(part ? v[4] / nn : 0.);
+    } else {
+        // raw second moments <u^2>, <v^2>, <w^2>, <uw>, <rho^2> into rows 3..6, 8
+        st[3 * NZ + slot] = part ? v[0] / nn : 0.;
+        st[4 * NZ + slot] = part ? v[1] / nn : 0.;
+        st[5 * NZ + slot] = part ? v[2] / nn : 0.;
+        st[6 * NZ + slot] = part ? v[3] / nn : 0.;
+        st[8 * NZ + slot] = part ? v[4] / nn : 0.;
+    }
+}
+
+static PlaneArgs plane_args(dnsg_ctx *h, const RedSet &R, int pass, double *stats, double *mc, double *ml,
+                            double *mu) {
+    PlaneArgs A;
+    A.pc = R.g[0].part; A.pl = R.g[1].part; A.pu = R.g[2].part;
+    A.Yc = h->d.Yc; A.Yf = h->d.Yf; A.zdc = h->d.zdc; A.zdf = h->d.zdf; A.NZ = h->d.NZ;
+    A.nc = (double)h->d.Xc * h->d.Yc; A.nf = (double)h->d.Xf * h->d.Yf;
+    A.pass = pass; A.stats = stats; A.means_c = mc; A.means_l = ml; A.means_u = mu;
+    return A;
+}
+
+// out9nz receives the 9 x NZ profiles (device pointer)
+hipError_t launch_statistics(hipStream_t s, dnsg_ctx *h, double *out9nz) {
+    double *mc = h->d_red, *ml = mc + 4 * h->coarse.nz, *mu = ml + 4 * h->lower.nz;
+    RedSet R1 = make_set(h, nullptr, nullptr, nullptr);
+    row_moments<1>(s, h->p.precision, R1, 3);
+    PlaneArgs A = plane_args(h, R1, 1, out9nz, mc, ml, mu);
+    hipLaunchKernelGGL(k_fin_planes, dim3(h->d.NZ), dim3(RB), 0, s, A);
+    RedSet R2 = make_set(h, mc, ml, mu);
+    row_moments<2>(s, h->p.precision, R2, 3);
+    A.pass = 2;
+    hipLaunchKernelGGL(k_fin_planes, dim3(h->d.NZ), dim3(RB), 0, s, A);
+    return hipGetLastError();
+}
+
+// Running averages: per snapshot, fold the exact instantaneous profiles
+// (two-pass, central moments) into 14 x NZ accumulators
+//   0..2  U, V, W        3..5  u_rms^2, v_rms^2, w_rms^2   6  u'w'
+//   7     rho - 1        8     rho_rms^2
+//   9..11 U^2, V^2, W^2  12    U W                       13  (rho - 1)^2
+// so that the time average of a second moment is the mean plane variance
+// plus the variance of the plane mean (law of total variance), with the
+// density kept as a deviation from 1 to avoid cancellation.
+__global__ void k_fold_stats(double *acc, const double *st, int NZ) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= NZ) return;
+    const double U = st[0 * NZ + k], V = st[1 * NZ + k], W = st[2 * NZ + k], D = st[7 * NZ + k] - 1.;
+    const double ur = st[3 * NZ + k], vr = st[4 * NZ + k], wr = st[5 * NZ + k], dr = st[8 * NZ + k];
+    acc[0 * NZ + k] += U;  acc[1 * NZ + k] += V;  acc[2 * NZ + k] += W;
+    acc[3 * NZ + k] += ur * ur; acc[4 * NZ + k] += vr * vr; acc[5 * NZ + k] += wr * wr;
+    acc[6 * NZ + k] += st[6 * NZ + k];
+    acc[7 * NZ + k] += D; acc[8 * NZ + k] += dr * dr;
+    acc[9 * NZ + k] += U * U; acc[10 * NZ + k] += V * V; acc[11 * NZ + k] += W * W;
+    acc[12 * NZ + k] += U * W; acc[13 * NZ + k] += D * D;
+}
+
+hipError_t launch_fold_stats(hipStream_t s, double *acc, const double *st, int NZ) {
+    hipLaunchKernelGGL(k_fold_stats, dim3((NZ + 255) / 256), dim3(256), 0, s, acc, st, NZ);
+    return hipGetLastError();
+}
+
+size_t partial_doubles(const dnsg_ctx *h) { return part_layout(h).total; }
+
+}  // namespace dnsg

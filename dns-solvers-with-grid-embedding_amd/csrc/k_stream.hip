@@ -1,0 +1,244 @@
+// k_stream.hip -- fused BGK collide + push-stream on the coarse grid and the
+// wall bands, the initial-state kernel and the AoS <-> SoA transposes.
+//
+// One workgroup owns one (y, z) row of a lattice and walks it in x, so every
+// write of the row -- including the periodic wrap x = nx-1 -> 0 -- lands in
+// cache lines that only this workgroup touches.  Reads are coalesced per
+// direction plane; writes are the same rows shifted by e_x.
+//
+// Boundary rules at the stored z ends (xfcomputations.c:215-442):
+//   BOUND_DROP  populations leaving the lattice are not written (the bounded
+//               layers of the reference: coarse k = 0, zdc-1 and the band
+//               interface layers, which the interface transfer rewrites)
+//   BOUND_WALL  the target is a wall node with code s(t) in {-1, -2}; with
+//               s1 the code below the source node:
+//                 s(t) = -1 and s1 = -1  -> halfway bounce-back  f(x, opp q)
+//                 otherwise              -> specular reflection  f(x + e_xy, mirror q)
+#include "internal.hpp"
+#include "lattice.hpp"
+
+namespace dnsg {
+
+template <class S>
+struct ComputeOf {
+    using T = float;
+};
+template <>
+struct ComputeOf<double> {
+    using T = double;
+};
+
+__device__ __forceinline__ void collide_any(double *f, double G, double om) { collide64(f, G, om); }
+__device__ __forceinline__ void collide_any(float *f, float G, float om) { collide32(f, G, om); }
+
+struct LatPair {
+    LatArgs l[2];
+};
+
+template <class S>
+__global__ __launch_bounds__(256) void k_collide_stream(LatPair P, const double *__restrict__ force) {
+    const LatArgs &L = P.l[blockIdx.y];
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int row = blockIdx.x;
+    if (row >= ny * nz) return;
+    using T = typename ComputeOf<S>::T;
+    const int y = row % ny, z = row / ny;
+    const size_t qs = (size_t)nx * ny * nz;
+    const S *__restrict__ cur = static_cast<const S *>(L.cur);
+    S *__restrict__ nxt = static_cast<S *>(L.nxt);
+    const T G = (T)(L.gscale * force[0]);
+    const T om = (T)L.omega;
+    const bool lo_edge = (z == 0), hi_edge = (z == nz - 1);
+    const int ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
+    const size_t row0 = ((size_t)z * ny + y) * nx;
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+        T f[Q];
+        const size_t base = row0 + x;
+        unroll<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            f[q] = load_s(cur + q * qs + base);
+        });
+        collide_any(f, G, om);
+        const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
+        unroll<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const int xt = (EX[q] > 0) ? xp : (EX[q] < 0 ? xm : x);
+            const int yt = (EY[q] > 0) ? yp : (EY[q] < 0 ? ym : y);
+            if constexpr (EZ[q] != 0) {
+                const bool at_end = (EZ[q] < 0) ? lo_edge : hi_edge;
+                if (at_end) {
+                    const int kind = (EZ[q] < 0) ? L.lo_kind : L.hi_kind;
+                    if (kind == BOUND_WALL) {
+                        const int8_t *wc = (EZ[q] < 0) ? L.wlo : L.whi;
+                        const int sld = wc[(size_t)yt * nx + xt];
+                        const int sld1 = wc[(size_t)y * nx + x];
+                        if (sld == -1 && sld1 == -1)
+                            store_s(nxt + OPP[q] * qs + base, f[q]);
+                        else
+                            store_s(nxt + MIR[q] * qs + row0 - (size_t)y * nx + (size_t)yt * nx + xt, f[q]);
+                    }
+                    return;
+                }
+            }
+            const size_t dst = ((size_t)(z + EZ[q]) * ny + yt) * nx + xt;
+            store_s(nxt + q * qs + dst, f[q]);
+        });
+    }
+}
+
+hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, const double *force, int prec) {
+    LatPair P;
+    int rows = 0;
+    for (int i = 0; i < 2; i++) P.l[i] = lats[i < nlat ? i : 0];
+    for (int i = 0; i < nlat; i++) rows = (lats[i].ny * lats[i].nz > rows) ? lats[i].ny * lats[i].nz : rows;
+    dim3 grid(rows, nlat), block(256);
+    if (prec == DNSG_PREC_FP64)
+        hipLaunchKernelGGL(k_collide_stream<double>, grid, block, 0, s, P, force);
+    else if (prec == DNSG_PREC_FP32)
+        hipLaunchKernelGGL(k_collide_stream<float>, grid, block, 0, s, P, force);
+    else
+        hipLaunchKernelGGL(k_collide_stream<__half>, grid, block, 0, s, P, force);
+    return hipGetLastError();
+}
+
+// ---- initial state --------------------------------------------------------------
+// Counter-based PRNG (splitmix64 finaliser) over (seed, grid, component, node):
+// independent of the traversal order, so every node's noise is fixed by its
+// global index in the reference layout [x][y][k] of that grid.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double prng_sym(uint64_t seed, int grid, int comp, uint64_t node) {
+    const uint64_t b =
+        mix64(mix64(seed) ^ mix64(((uint64_t)(grid & 0xff) << 56) ^ ((uint64_t)(comp & 0xff) << 48) ^ node));
+    const double u = (double)(b >> 11) * 0x1.0p-53;
+    return 2. * u - 1.;
+}
+
+struct InitArgs {
+    void *dst;
+    int nx, ny, nz;     // stored extents
+    int zdf_or_zdc;     // reference z extent of this grid
+    int kofs;           // reference k = stored z + kofs
+    int grid;           // 0 coarse, 1 lower, 2 upper (PRNG stream)
+    int kind;           // DNSG_INIT_*
+    int GR, zdf, zdc, NZs;
+    uint64_t seed;
+    double u_max, noise;
+};
+
+template <class S>
+__global__ __launch_bounds__(256) void k_init(InitArgs A) {
+    const size_t n = (size_t)A.nx * A.ny * A.nz;
+    const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const int x = (int)(id % A.nx), y = (int)((id / A.nx) % A.ny), z = (int)(id / ((size_t)A.nx * A.ny));
+    const int k = z + A.kofs;
+    S *dst = static_cast<S *>(A.dst);
+    double f[Q];
+    if (A.kind == DNSG_INIT_REST) {
+        unroll<0, Q>([&](auto qc) { f[decltype(qc)::value] = WQ[decltype(qc)::value]; });
+    } else {
+        double dist;
+        if (A.grid == 0) dist = A.GR * (k - 1) + (A.zdf - 1 - 0.5);
+        else if (A.grid == 1) dist = k - 0.5;
+        else dist = k + A.GR * (A.zdc - 3) + A.zdf - 1. - 0.5;
+        const double zeta = dist / ((double)(A.NZs - 2));
+        const double amp = A.noise * A.u_max;
+        const uint64_t node = ((uint64_t)x * A.ny + y) * A.zdf_or_zdc + k;
+        const double u0 = 4. * A.u_max * zeta * (1. - zeta) + amp * prng_sym(A.seed, A.grid, 0, node);
+        const double u1 = amp * prng_sym(A.seed, A.grid, 1, node);
+        const double u2 = amp * prng_sym(A.seed, A.grid, 2, node);
+        const double v = u0 * u0 + u1 * u1 + u2 * u2;
+        unroll<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            f[q] = feq64<q>(1., u0, u1, u2, v);
+        });
+    }
+    const size_t qs = n;
+    unroll<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (sizeof(S) == 8) store_s((double *)dst + q * qs + id, f[q]);
+        else store_s(dst + q * qs + id, (float)(f[q] - WQ[q]));
+    });
+}
+
+hipError_t launch_init_lattice(hipStream_t s, dnsg_ctx *h, int grid, void *dst) {
+    InitArgs A;
+    const Grid &g = grid == 0 ? h->coarse : (grid == 1 ? h->lower : h->upper);
+    A.dst = dst;
+    A.nx = g.nx; A.ny = g.ny; A.nz = g.nz;
+    A.zdf_or_zdc = grid == 0 ? h->d.zdc : h->d.zdf;
+    A.kofs = grid == 1 ? 1 : 0;
+    A.grid = grid;
+    A.kind = h->p.init_kind;
+    A.GR = h->d.GR; A.zdf = h->d.zdf; A.zdc = h->d.zdc; A.NZs = h->d.NZs;
+    A.seed = h->p.seed; A.u_max = h->p.u_max; A.noise = h->p.noise;
+    const size_t n = g.nodes();
+    dim3 grd((unsigned)((n + 255) / 256)), blk(256);
+    if (h->p.precision == DNSG_PREC_FP64) hipLaunchKernelGGL(k_init<double>, grd, blk, 0, s, A);
+    else if (h->p.precision == DNSG_PREC_FP32) hipLaunchKernelGGL(k_init<float>, grd, blk, 0, s, A);
+    else hipLaunchKernelGGL(k_init<__half>, grd, blk, 0, s, A);
+    return hipGetLastError();
+}
+
+// ---- AoS (reference file layout [x][y][k][19]) <-> SoA -----------------------------
+template <class S>
+__global__ __launch_bounds__(256) void k_aos_to_soa(const double *__restrict__ aos, S *__restrict__ soa, int nx,
+                                                    int ny, int nz_aos, int z0, int nz) {
+    const size_t qs = (size_t)nx * ny * nz;
+    const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= qs * Q) return;
+    const int q = (int)(id / qs);
+    const size_t r = id % qs;
+    const int x = (int)(r % nx), y = (int)((r / nx) % ny), z = (int)(r / ((size_t)nx * ny));
+    const double v = aos[(((size_t)x * ny + y) * nz_aos + (z + z0)) * Q + q];
+    if constexpr (sizeof(S) == 8) soa[id] = v;
+    else store_s(soa + id, (float)(v - WQ[q]));
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_soa_to_aos(const S *__restrict__ soa, double *__restrict__ aos, int nx,
+                                                    int ny, int nz_aos, int z0, int nz) {
+    const size_t qs = (size_t)nx * ny * nz;
+    const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= qs * Q) return;
+    const int q = (int)(id / qs);
+    const size_t r = id % qs;
+    const int x = (int)(r % nx), y = (int)((r / nx) % ny), z = (int)(r / ((size_t)nx * ny));
+    double v;
+    if constexpr (sizeof(S) == 8) v = soa[id];
+    else v = (double)load_s(soa + id) + WQ[q];
+    aos[(((size_t)x * ny + y) * nz_aos + (z + z0)) * Q + q] = v;
+}
+
+hipError_t launch_aos_to_soa(hipStream_t s, const double *aos, void *soa, int nx, int ny, int nz_aos, int z0, int nz,
+                             int prec) {
+    const size_t n = (size_t)nx * ny * nz * Q;
+    dim3 grd((unsigned)((n + 255) / 256)), blk(256);
+    if (prec == DNSG_PREC_FP64)
+        hipLaunchKernelGGL(k_aos_to_soa<double>, grd, blk, 0, s, aos, (double *)soa, nx, ny, nz_aos, z0, nz);
+    else if (prec == DNSG_PREC_FP32)
+        hipLaunchKernelGGL(k_aos_to_soa<float>, grd, blk, 0, s, aos, (float *)soa, nx, ny, nz_aos, z0, nz);
+    else
+        hipLaunchKernelGGL(k_aos_to_soa<__half>, grd, blk, 0, s, aos, (__half *)soa, nx, ny, nz_aos, z0, nz);
+    return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(hipStream_t s, const void *soa, double *aos, int nx, int ny, int nz_aos, int z0, int nz,
+                             int prec) {
+    const size_t n = (size_t)nx * ny * nz * Q;
+    dim3 grd((unsigned)((n + 255) / 256)), blk(256);
+    if (prec == DNSG_PREC_FP64)
+        hipLaunchKernelGGL(k_soa_to_aos<double>, grd, blk, 0, s, (const double *)soa, aos, nx, ny, nz_aos, z0, nz);
+    else if (prec == DNSG_PREC_FP32)
+        hipLaunchKernelGGL(k_soa_to_aos<float>, grd, blk, 0, s, (const float *)soa, aos, nx, ny, nz_aos, z0, nz);
+    else
+        hipLaunchKernelGGL(k_soa_to_aos<__half>, grd, blk, 0, s, (const __half *)soa, aos, nx, ny, nz_aos, z0, nz);
+    return hipGetLastError();
+}
+
+}  // namespace dnsg

@@ -1,0 +1,464 @@
+/*
+ * dnsg_driver.c -- the kept C driver: the reference's main() (dns-q-mgrd-all-
+ * patterns-xyc.c:15-1535) on one GPU through the dnsg_* C-ABI.
+ *
+ *   dnsg-run -s|-p [definitions.h] [key=value ...]
+ *
+ * Parameters come from a definitions.h-format file (the reference's own
+ * `#define NAME value` lines are parsed, including the RSLIP/PSLIP/SSLIP and
+ * ITER switches) and/or key=value arguments with the dnsg_params field names,
+ * which also reach the runtime-only extensions (force_mode, g_const, tau_c,
+ * precision, init_kind, seed, u_max, noise, results_dat).
+ *
+ * Run modes (main:944-1190): -s starts from rest ('-s' of the reference) or,
+ * with init_kind=1, from the seeded laminar profile with noise; -p reads the
+ * restart files lbf-c/slf/suf/lin/uin.TSTR and force.TSTR from the working
+ * directory (their sizes are validated, which the reference does not do).
+ *
+ * Output files, written to the working directory in the reference's names and
+ * layouts: flowrate.txt, density.txt, str-up.txt, str-do.txt, w-str.txt,
+ * p-grad.txt (every step); vel-{c,slf,suf}.NNNN, den-*.NNNN, turb-fld.000.NNN,
+ * uavg-temp.dat, uavg-exact-lam.dat (every utime steps); lbf-*.NNNN and
+ * force.NNNN (every 50 utime steps); 2dresults-exact.dat, 2dresults-lb.dat,
+ * 2dresults-lb.bdat, report.code and results.dat at the end.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <ctype.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <time.h>
+
+#include "../../include/dnsg.h"
+
+static const double E[3][19] = {{0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, 1, -1, -1, 0, 0, 0, 0},
+                                {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, 1, -1, -1},
+                                {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1}};
+
+static int results_dat = 1;
+
+static double now(void) {
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return t.tv_sec + 1e-9 * t.tv_nsec;
+}
+
+static void die(dnsg_t *h, const char *what, int rc) {
+    fprintf(stderr, "%s failed (%d): %s\n", what, rc, h ? dnsg_last_error(h) : "");
+    exit(1);
+}
+
+/* one NAME=value assignment (definitions.h #define or command line) */
+static int set_param(dnsg_params *p, const char *key, const char *val) {
+#define D(n) if (!strcmp(key, #n)) { p->n = strtod(val, NULL); return 0; }
+#define I(n) if (!strcmp(key, #n)) { p->n = (int)strtol(val, NULL, 10); return 0; }
+    D(Rebs) D(ub) I(zdv) I(ydv) I(xdv) I(TSTR) I(TEND) D(CFL) I(gratio) I(zds)
+    I(gyplussize) I(wyplussize) I(gxplussize) I(wxplussize) I(pattern)
+    I(force_mode) D(g_const) D(tau_c) I(precision) I(init_kind) D(u_max) D(noise)
+#undef D
+#undef I
+    if (!strcmp(key, "ITER")) { p->ITER = strtol(val, NULL, 10); return 0; }
+    if (!strcmp(key, "seed")) { p->seed = strtoull(val, NULL, 10); return 0; }
+    if (!strcmp(key, "results_dat")) { results_dat = atoi(val); return 0; }
+    if (!strcmp(key, "XDIM") || !strcmp(key, "YDIM")) return 0; /* one GPU = one rank */
+    return -1;
+}
+
+/* parse a definitions.h-format file: active `#define NAME value` lines */
+static int read_definitions(dnsg_params *p, const char *path) {
+    FILE *f = fopen(path, "r");
+    char line[512];
+    if (!f) return -1;
+    p->pattern = DNSG_PATTERN_NONE;
+    p->ITER = 0;
+    while (fgets(line, sizeof line, f)) {
+        char name[128], val[256];
+        const char *s = line;
+        while (isspace((unsigned char)*s)) s++;
+        if (strncmp(s, "#define", 7)) continue;
+        val[0] = 0;
+        if (sscanf(s + 7, "%127s %255s", name, val) < 1) continue;
+        if (!strcmp(name, "RSLIP")) p->pattern = DNSG_PATTERN_RSLIP;
+        else if (!strcmp(name, "PSLIP")) p->pattern = DNSG_PATTERN_PSLIP;
+        else if (!strcmp(name, "SSLIP")) p->pattern = DNSG_PATTERN_SSLIP;
+        else if (val[0]) set_param(p, name, val);
+    }
+    fclose(f);
+    return 0;
+}
+
+static int read_doubles(const char *fn, double *dst, size_t n, size_t expect_total) {
+    FILE *f = fopen(fn, "rb");
+    struct stat st;
+    if (!f) { fprintf(stderr, "cannot open %s\n", fn); return -1; }
+    if (fstat(fileno(f), &st) || (size_t)st.st_size != expect_total * 8) {
+        fprintf(stderr, "%s: size %ld, expected %zu bytes\n", fn, (long)st.st_size, expect_total * 8);
+        fclose(f);
+        return -1;
+    }
+    if (fread(dst, 8, n, f) != n) { fclose(f); return -1; }
+    fclose(f);
+    return 0;
+}
+
+static int write_file(const char *fn, const double *a, size_t n, const double *b, size_t m) {
+    FILE *f = fopen(fn, "wb");
+    if (!f) return -1;
+    fwrite(a, 8, n, f);
+    if (b) fwrite(b, 8, m, f);
+    return fclose(f);
+}
+
+static void append_log(const char *fn, double t, double v) {
+    FILE *f = fopen(fn, "a");
+    if (!f) return;
+    fprintf(f, "%f %.12f\n", t, v);
+    fclose(f);
+}
+
+/* checkpoint (main:1262-1341): ftemp then f for every lattice; the second half
+ * of the reference's files is a scratch buffer that the restart never reads, it
+ * holds the state again here */
+static void checkpoint(dnsg_t *h, const dnsg_dims *d, long idx, double force) {
+    double *c = malloc(d->n_coarse * 8), *lo = malloc(d->n_band * 8), *up = malloc(d->n_band * 8);
+    double *li = malloc(d->n_plane * 8), *ui = malloc(d->n_plane * 8);
+    char fn[64];
+    int rc;
+    if (!c || !lo || !up || !li || !ui) { fprintf(stderr, "checkpoint: out of memory\n"); exit(1); }
+    if ((rc = dnsg_download_state(h, c, lo, up, li, ui, NULL))) die(h, "dnsg_download_state", rc);
+    snprintf(fn, sizeof fn, "lbf-c.%.4ld", idx);   write_file(fn, c, d->n_coarse, c, d->n_coarse);
+    snprintf(fn, sizeof fn, "lbf-slf.%.4ld", idx); write_file(fn, lo, d->n_band, lo, d->n_band);
+    snprintf(fn, sizeof fn, "lbf-suf.%.4ld", idx); write_file(fn, up, d->n_band, up, d->n_band);
+    snprintf(fn, sizeof fn, "force.%.4ld", idx);   write_file(fn, &force, 1, NULL, 0);
+    snprintf(fn, sizeof fn, "lbf-uin.%.4ld", idx); write_file(fn, ui, d->n_plane, NULL, 0);
+    snprintf(fn, sizeof fn, "lbf-lin.%.4ld", idx); write_file(fn, li, d->n_plane, NULL, 0);
+    printf("writing V.force=%.12f num written=1 myrank=0\n", force);
+    free(c); free(lo); free(up); free(li); free(ui);
+}
+
+/* vel_print.c:15-274 */
+static void vel_print(dnsg_t *h, const dnsg_dims *d, long cntr) {
+    static const char *tag[3] = {"c", "slf", "suf"};
+    static const int order[3] = {0, 2, 1};
+    int gi;
+    for (gi = 0; gi < 3; gi++) {
+        const int g = order[gi];
+        const size_t n = g == 0 ? (size_t)d->Xc * d->Yc * d->zdc : (size_t)d->Xf * d->Yf * d->zdf;
+        double *vel = malloc(n * 3 * 8), *den = malloc(n * 8);
+        char fn[64];
+        int rc;
+        if (!vel || !den) { fprintf(stderr, "vel_print: out of memory\n"); exit(1); }
+        if ((rc = dnsg_velocity_snapshot(h, g, vel, den))) die(h, "dnsg_velocity_snapshot", rc);
+        snprintf(fn, sizeof fn, "vel-%s.%.4d", tag[g], (int)cntr); write_file(fn, vel, n * 3, NULL, 0);
+        snprintf(fn, sizeof fn, "den-%s.%.4d", tag[g], (int)cntr); write_file(fn, den, n, NULL, 0);
+        free(vel); free(den);
+    }
+}
+
+/* wall-normal coordinates of the profile slots (statistics.c:319-329) */
+static void profile_coords(const dnsg_dims *d, double *zps) {
+    const int zdf = d->zdf, zdc = d->zdc, GR = d->GR, NZ = d->NZ;
+    int k;
+    memset(zps, 0, sizeof(double) * NZ);
+    for (k = 1; k < zdf; k++) zps[k] = k - 0.5;
+    for (k = 2; k < (zdc - 2); k++) zps[zdf - 2 + k] = GR * (k - 1) + (zdf - 1 - 0.5);
+    for (k = 0; k < (zdf - 1); k++) zps[zdf + zdc - 4 + k] = k + GR * (zdc - 3) + zdf - 1. - 0.5;
+    zps[2 * zdf + zdc - 5] = 2 * zdf + GR * (zdc - 4) + GR - 3;
+    for (k = 0; k < NZ; k++) zps[k] /= ((double)(d->NZs - 2.));
+}
+
+/* statistics.c:303-357 file part */
+static void statistics(dnsg_t *h, const dnsg_dims *d, const dnsg_params *p, long ts) {
+    const int NZ = d->NZ, NZs = d->NZs;
+    double *st = malloc(sizeof(double) * 9 * NZ), *zps = malloc(sizeof(double) * NZ);
+    double *y = calloc(NZs, sizeof(double)), *zpsf = calloc(NZs, sizeof(double));
+    char fn[64];
+    FILE *f;
+    int k, rc;
+    if ((rc = dnsg_statistics(h, st))) die(h, "dnsg_statistics", rc);
+    snprintf(fn, sizeof fn, "turb-fld.%.3d.%.3ld", 0, ts);
+    write_file(fn, st, (size_t)9 * NZ, NULL, 0);
+    profile_coords(d, zps);
+    for (k = 1; k < (NZs - 1); k++)
+        y[k] = -6. * p->ub * ((k - 0.5) / (NZs - 2.)) * ((k - 0.5) / (NZs - 2.) - 1);
+    if ((f = fopen("uavg-temp.dat", "w"))) {
+        for (k = 0; k < NZ; k++) fprintf(f, "%.12f %.12f\n", zps[k], st[k]);
+        fclose(f);
+    }
+    for (k = 1; k < (NZs - 1); k++) zpsf[k] = k - 0.5;
+    zpsf[NZs - 1] = NZs - 2;
+    for (k = 0; k < NZs; k++) zpsf[k] /= ((double)(NZs - 2.));
+    if ((f = fopen("uavg-exact-lam.dat", "w"))) {
+        for (k = 0; k < NZs; k++) fprintf(f, "%.12f %.12f \n", zpsf[k], y[k]);
+        fclose(f);
+    }
+    free(st); free(zps); free(y); free(zpsf);
+}
+
+static double node_ux(const double *f, double *dens) {
+    double ux = 0., d = 0.;
+    int a;
+    for (a = 0; a < 19; a++) { d += f[a]; ux += E[0][a] * f[a]; }
+    *dens = d;
+    return ux;
+}
+
+/* output_twoD2.c:15-169 (the files it writes replace output_twoD's) */
+static void output_twoD2(const dnsg_dims *d, const dnsg_params *p, const double *c, const double *lo,
+                         const double *up) {
+    const int GR = d->GR, zdf = d->zdf, zdc = d->zdc, NZs = d->NZs, Xf = d->Xf, Yf = d->Yf, Xc = d->Xc, Yc = d->Yc;
+    const int NZ = 2 * (zdf - GR - 1) + zdc;
+    double *jj = calloc(NZ, 8), *zps = calloc(NZ, 8), *y = calloc(NZs, 8), *zpsf = calloc(NZs, 8);
+    int i, j, k;
+    FILE *f;
+    for (k = 1; k < (NZs - 1); k++) zpsf[k] = k - 0.5;
+    zpsf[NZs - 1] = NZs - 2;
+    for (k = 0; k < NZs; k++) zpsf[k] /= ((double)(NZs - 2.));
+    for (k = 1; k < (zdf - GR - 1); k++) zps[k] = k - 0.5;
+    for (k = 0; k < zdc; k++) zps[zdf - GR - 1 + k] = GR * (k) + (zdf - GR - 1 - 0.5);
+    for (k = GR + 1; k < (zdf - 1); k++) zps[zdf + zdc - GR - 2 - GR + k] = k - GR + GR * (zdc - 1) + zdf - GR - 1. - 0.5;
+    zps[NZ - 1] = 2 * zdf + GR * (zdc - 4) + GR - 3;
+    for (k = 0; k < NZ; k++) zps[k] /= ((double)(NZs - 2.));
+    for (k = 1; k < (NZs - 1); k++) y[k] = -6. * p->ub * ((k - 0.5) / (NZs - 2.)) * ((k - 0.5) / (NZs - 2.) - 1);
+    for (i = 0; i < Xf; i++)
+        for (j = 0; j < Yf; j++)
+            for (k = 1; k < (zdf - GR - 1); k++) {
+                double dens, ux = node_ux(lo + (((size_t)i * Yf + j) * zdf + k) * 19, &dens);
+                jj[k] += ux / dens;
+            }
+    for (k = 1; k < (zdf - GR - 1); k++) jj[k] /= (double)((Xf + 0.) * 1 * (Yf + 0.) * 1);
+    for (i = 0; i < Xc; i++)
+        for (j = 0; j < Yc; j++)
+            for (k = 0; k < zdc; k++) {
+                double dens, ux = node_ux(c + (((size_t)i * Yc + j) * zdc + k) * 19, &dens);
+                jj[zdf - GR - 1 + k] += ux / dens;
+            }
+    for (k = 0; k < zdc; k++) jj[zdf - GR - 1 + k] /= (double)((Xc + 0.) * 1 * (Yc + 0.) * 1);
+    for (i = 0; i < Xf; i++)
+        for (j = 0; j < Yf; j++)
+            for (k = GR + 1; k < (zdf - 1); k++) {
+                double dens, ux = node_ux(up + (((size_t)i * Yf + j) * zdf + k) * 19, &dens);
+                jj[zdf + zdc - GR - 2 - GR + k] += ux / dens;
+            }
+    for (k = GR + 1; k < (zdf - 1); k++) jj[zdf + zdc - GR - 2 - GR + k] /= (double)((Xf + 0.) * 1 * (Yf + 0.) * 1);
+    if ((f = fopen("2dresults-exact.dat", "w"))) {
+        for (k = 0; k < NZs; k++) fprintf(f, "%.12f %.12f \n", zpsf[k], y[k]);
+        fclose(f);
+    }
+    if ((f = fopen("2dresults-lb.dat", "w"))) {
+        for (k = 0; k < NZ; k++) fprintf(f, "%.12f %.12f\n", zps[k], jj[k]);
+        fclose(f);
+    }
+    write_file("2dresults-lb.bdat", jj, NZ, NULL, 0);
+    free(jj); free(zps); free(y); free(zpsf);
+}
+
+/* output.c:15-107 (one rank): Tecplot points of the three grids */
+static void output_tecplot(const dnsg_dims *d, const dnsg_params *p, const double *c, const double *lo,
+                           const double *up) {
+    const int GR = d->GR, zdf = d->zdf, zdc = d->zdc, Xf = d->Xf, Yf = d->Yf, Xc = d->Xc, Yc = d->Yc;
+    const double h = (d->NZs - 2.) / 2., ub = p->ub;
+    double *zps = malloc(sizeof(double) * d->NZ);
+    FILE *f = fopen("results.dat", "w");
+    int i, j, k, a;
+    if (!f) { free(zps); return; }
+    {   /* unnormalised slot coordinates (output.c:25-33) */
+        dnsg_dims t = *d;
+        t.NZs = 3; /* profile_coords divides by NZs-2: keep the raw values */
+        profile_coords(&t, zps);
+    }
+    fprintf(f, "TITLE = \"3d velocity field\"\n");
+    fprintf(f, "VARIABLES = \"x/h\", \"y/h\", \"z/h\", \"rho\", \"u\", \"v\", \"w\" \n");
+    fprintf(f, "ZONE I=%d, J=%d, K=%d, F=POINT", Xf, Yf + 1, zdf);
+#define NODE(arr, Y, Z, ii, jj, kk)                                                                    \
+    do {                                                                                               \
+        const double *fp = (arr) + (((size_t)(ii) * (Y) + (jj)) * (Z) + (kk)) * 19;                    \
+        ux = 0.; uy = 0.; uz = 0.; dens = 0.;                                                          \
+        for (a = 0; a < 19; a++) { ux += E[0][a] * fp[a]; uy += E[1][a] * fp[a]; uz += E[2][a] * fp[a]; dens += fp[a]; } \
+    } while (0)
+    for (k = 0; k < zdf; k++)
+        for (j = 0; j < Yf + 1; j++)
+            for (i = 0; i < Xf; i++) {
+                double ux, uy, uz, dens;
+                NODE(lo, Yf, zdf, i, j % Yf, k);
+                fprintf(f, "%f %f %f %f %f %f %f\n", (i + 0.5) / h, (j + 0.5) / h, (k - 0.5) / h, dens, ux / ub, uy / ub, uz / ub);
+            }
+    fprintf(f, "ZONE I=%d, J=%d, K=%d, F=POINT", Xc, Yc + 1, zdc - 4);
+    for (k = 2; k < (zdc - 2); k++)
+        for (j = 0; j < Yc + 1; j++)
+            for (i = 0; i < Xc; i++) {
+                double ux, uy, uz, dens;
+                NODE(c, Yc, zdc, i, j % Yc, k);
+                fprintf(f, "%f %f %f %f %f %f %f\n", (i * GR + 0.5) / h, (j * GR + 0.5) / h, zps[zdf - 2 + k] / h, dens, ux / ub, uy / ub, uz / ub);
+            }
+    fprintf(f, "ZONE I=%d, J=%d, K=%d, F=POINT", Xf, Yf + 1, zdf);
+    for (k = 0; k < zdf; k++)
+        for (j = 0; j < Yf + 1; j++)
+            for (i = 0; i < Xf; i++) {
+                double ux, uy, uz, dens;
+                NODE(up, Yf, zdf, i, j % Yf, k);
+                fprintf(f, "%f %f %f %f %f %f %f\n", (i + 0.5) / h, (j + 0.5) / h, zps[zdf + zdc - 4 + k] / h, dens, ux / ub, uy / ub, uz / ub);
+            }
+#undef NODE
+    fprintf(f, "\n\n");
+    fclose(f);
+    free(zps);
+}
+
+int main(int argc, char **argv) {
+    dnsg_params p;
+    dnsg_dims d;
+    dnsg_t *h = NULL;
+    int i, rc;
+    char mode;
+    long ss, TM, up, ut;
+    double t4, t2, mass;
+    if (argc < 2 || argv[1][0] != '-' || !argv[1][1]) {
+        fprintf(stderr, "usage: %s -s|-p [definitions.h] [key=value ...]\n", argv[0]);
+        return 2;
+    }
+    mode = argv[1][1];
+    dnsg_params_default(&p);
+    for (i = 2; i < argc; i++) {
+        const char *eq = strchr(argv[i], '=');
+        if (!eq) {
+            if (read_definitions(&p, argv[i])) { fprintf(stderr, "cannot read %s\n", argv[i]); return 2; }
+        } else {
+            char key[128];
+            size_t n = (size_t)(eq - argv[i]);
+            if (n >= sizeof key) return 2;
+            memcpy(key, argv[i], n);
+            key[n] = 0;
+            if (set_param(&p, key, eq + 1)) { fprintf(stderr, "unknown parameter %s\n", key); return 2; }
+        }
+    }
+    if ((rc = dnsg_create(&p, &h))) die(h, "dnsg_create", rc);
+    dnsg_get_dims(h, &d);
+    printf("%s: is running the process 0 of total 1 processors\n", "dnsg-run");
+    switch (mode) {
+    case 's':
+        TM = p.TSTR;
+        if ((rc = dnsg_init_state(h))) die(h, "dnsg_init_state", rc);
+        break;
+    case 'p': {
+        char fn[64];
+        double *c = malloc(d.n_coarse * 8), *lo = malloc(d.n_band * 8), *upb = malloc(d.n_band * 8);
+        double *li = malloc(d.n_plane * 8), *ui = malloc(d.n_plane * 8), force = 0.;
+        const long cntr = p.TSTR;
+        fprintf(stderr, "TRYING TO READ DATA FROM FILES\n");
+        if (!c || !lo || !upb || !li || !ui) { fprintf(stderr, "out of memory\n"); return 1; }
+        snprintf(fn, sizeof fn, "lbf-c.%.4ld", cntr);
+        if (read_doubles(fn, c, d.n_coarse, 2 * d.n_coarse)) return 1;
+        snprintf(fn, sizeof fn, "lbf-slf.%.4ld", cntr);
+        if (read_doubles(fn, lo, d.n_band, 2 * d.n_band)) return 1;
+        snprintf(fn, sizeof fn, "lbf-suf.%.4ld", cntr);
+        if (read_doubles(fn, upb, d.n_band, 2 * d.n_band)) return 1;
+        snprintf(fn, sizeof fn, "force.%.4ld", cntr);
+        if (read_doubles(fn, &force, 1, 1)) return 1;
+        snprintf(fn, sizeof fn, "lbf-uin.%.4ld", cntr);
+        if (read_doubles(fn, ui, d.n_plane, d.n_plane)) return 1;
+        snprintf(fn, sizeof fn, "lbf-lin.%.4ld", cntr);
+        if (read_doubles(fn, li, d.n_plane, d.n_plane)) return 1;
+        printf("reading V.force=%.12f myrank=0\n", force);
+        if ((rc = dnsg_upload_state(h, c, lo, upb, li, ui, force))) die(h, "dnsg_upload_state", rc);
+        free(c); free(lo); free(upb); free(li); free(ui);
+        fprintf(stderr, "END OF READING DATA FROM FILES\nSTARTING FROM %ld\n", cntr);
+        TM = cntr;
+        break;
+    }
+    case 'i':
+        fprintf(stderr, "the '-i' uniform-grid initialiser is not part of this build\n");
+        return 2;
+    default:
+        fprintf(stderr, "not a good file name\n");
+        return 1;
+    }
+    if ((rc = dnsg_mass(h, &mass))) die(h, "dnsg_mass", rc);
+    printf("Average Density in the channel at start: %.12f  myrank=0\n", mass);
+
+    up = d.up;
+    ut = d.utime;
+    t4 = now();
+    ss = 0;
+    while (ss < up + 1) { /* main:1254-1375 */
+        long next = up + 1, n, s;
+        dnsg_diag *dg;
+        if (!(ss % ut)) {
+            /* output due at the top of this step, after encalc2/wshearstr */
+            dnsg_diag d0;
+            if ((rc = dnsg_encalc2(h, ss, &d0))) die(h, "dnsg_encalc2", rc);
+            if (!(ss % (50 * ut))) checkpoint(h, &d, TM + (ss / ut), d0.force);
+            vel_print(h, &d, TM + (ss / ut));
+            statistics(h, &d, &p, TM + (ss / ut));
+        }
+        /* run up to the next output step (or the end) in one batch */
+        next = ((ss / ut) + 1) * ut;
+        if (next > up + 1) next = up + 1;
+        n = next - ss;
+        dg = malloc(sizeof(dnsg_diag) * (size_t)n);
+        rc = dnsg_step(h, (int)n, dg);
+        for (s = 0; s < n; s++) {
+            if (rc == DNSG_ERR_NAN && isnan(dg[s].stress)) {
+                append_log("flowrate.txt", dg[s].time, dg[s].flowrate);
+                append_log("density.txt", dg[s].time, dg[s].density);
+                fprintf(stderr, "Error: NAN in wal shear stress\n");
+                return 1;
+            }
+            append_log("flowrate.txt", dg[s].time, dg[s].flowrate);
+            append_log("density.txt", dg[s].time, dg[s].density);
+            append_log("str-up.txt", dg[s].time, dg[s].str_up);
+            append_log("str-do.txt", dg[s].time, dg[s].str_down);
+            append_log("w-str.txt", dg[s].time, dg[s].stress);
+            append_log("p-grad.txt", dg[s].time, dg[s].pgrad);
+        }
+        free(dg);
+        if (rc) die(h, "dnsg_step", rc);
+        ss = next;
+    }
+    t2 = now();
+    {   /* end-of-run outputs (main:1377-1414) */
+        double *c = malloc(d.n_coarse * 8), *lo = malloc(d.n_band * 8), *upb = malloc(d.n_band * 8);
+        FILE *tm;
+        if (!c || !lo || !upb) { fprintf(stderr, "out of memory\n"); return 1; }
+        if ((rc = dnsg_download_state(h, c, lo, upb, NULL, NULL, NULL))) die(h, "dnsg_download_state", rc);
+        output_twoD2(&d, &p, c, lo, upb);
+        if ((tm = fopen("report.code", "w"))) {
+            fprintf(tm, "xdc=%d, ydc=%d, zdc+2=%d, Reb=%f, tauc=%f, G=%.10f, Ub=%f, CFLnumber=%f\n", d.Xc + 1,
+                    d.Yc + 1, d.zdc, p.Rebs, d.omega_c, 0., p.ub, p.CFL);
+            fprintf(tm, "xdf=%d, ydf=%d, zdf+2=%d, Reb=%f, tauf=%f, G=%.10f, Ub=%f, CFLnumber=%f\n", d.Xf + 1,
+                    d.Yf + 1, d.zdf, p.Rebs, d.omega_f, 0., p.ub, p.CFL);
+            fprintf(tm, "NZ fine +2 = %d   GR=%d \n", d.NZs, d.GR);
+            fprintf(tm, "number of processors: %d\n", 1);
+            fprintf(tm, "Executed File's Name: %s\n", __FILE__);
+            fprintf(tm, "Compilation Date: %s\n", __DATE__);
+            fprintf(tm, "Compilation Time: %s\n", __TIME__);
+            fprintf(tm, "Compiled wit standard C Compiler? (1 means yes): %d\n", __STDC__);
+            fprintf(tm, "Compilation Option: xcut\n");
+            if (p.pattern == DNSG_PATTERN_RSLIP) fprintf(tm, "slip option: RSLIP\n");
+            if (p.pattern == DNSG_PATTERN_PSLIP) fprintf(tm, "slip option: PSLIP\n");
+            if (p.pattern == DNSG_PATTERN_SSLIP) fprintf(tm, "slip option: SSLIP\n");
+            fprintf(tm, "Total computational time is: %f s\n", t2 - t4);
+            fprintf(tm, "Unit non-dimensional time is: %f iterations\n", (double)ut);
+            fprintf(tm, "Starting time: %d\n", p.TSTR);
+            fprintf(tm, "Current Time is: %d\n", p.TEND);
+            fprintf(tm, "size of lbf-c: %lu lbf-f: %lu\n", (unsigned long)(d.n_coarse * 2 * 8), (unsigned long)(d.n_band * 2 * 8));
+            fprintf(tm, "size of vel-c: %lu vel-f: %lu\n", (unsigned long)(d.n_coarse / 19 * 3 * 8), (unsigned long)(d.n_band / 19 * 3 * 8));
+            fprintf(tm, "size of den-c: %lu den-f: %lu\n", (unsigned long)(d.n_coarse / 19 * 8), (unsigned long)(d.n_band / 19 * 8));
+            fprintf(tm, "size of double: %lu\n", (unsigned long)sizeof(double));
+            fprintf(tm, "size of uin %lu\n", (unsigned long)(d.n_plane * 8));
+            fclose(tm);
+        }
+        if (results_dat) output_tecplot(&d, &p, c, lo, upb);
+        free(c); free(lo); free(upb);
+    }
+    statistics(h, &d, &p, TM + (ss / ut));
+    if ((rc = dnsg_mass(h, &mass))) die(h, "dnsg_mass", rc);
+    printf("Average Density in the channel at end: %.12f  myrank=0\n", mass);
+    printf("GLUPS: %.4f over %ld steps (%.3f s)\n",
+           (double)(up + 1) * ((double)d.Xc * d.Yc * d.zdc + d.GR * 2. * d.Xf * d.Yf * (d.zdf - 1)) / (t2 - t4) / 1e9,
+           up + 1, t2 - t4);
+    dnsg_destroy(h);
+    return 0;
+}

@@ -189,6 +189,15 @@ int dnsg_step(dnsg_t *h, int n, dnsg_diag *diag);
  * the n steps (device time, inputs resident in HBM). */
 int dnsg_run_timed(dnsg_t *h, int n, dnsg_diag *diag, double *elapsed_ms);
 
+/* Instrumentation (no counterpart in the reference, which only times the whole
+ * loop with MPI_Wtime, main:1252,1377): run n steps with HIP events between the
+ * kernel groups on the handle's stream and return the average device time in ms
+ * per launch of each group: ms[0] reductions (encalc2 + wshearstr),
+ * ms[1] coarse collide-stream, ms[2] interface sampling, ms[3] band
+ * collide-stream (both bands, one launch per substep), ms[4] interface fill
+ * (+ fine->coarse injection after the last substep). */
+int dnsg_kernel_times(dnsg_t *h, int n, double *ms);
+
 /* ---- reference-granular entry points (one per fun_defs.h function) ---------- */
 
 /* encalc2.c:17-147 -> V.force[0] (flux mode) and the flowrate/density scalars. */
@@ -217,8 +226,10 @@ int dnsg_mass(dnsg_t *h, double *avg_density);
  * of turb-fld.000.NNN: U, V, W, u_rms, v_rms, w_rms, u'w', rho, rho_rms. */
 int dnsg_statistics(dnsg_t *h, double *out);
 
-/* Running (time-averaged) statistics accumulated on the device in FP64: adds the
- * current plane means and second moments to the accumulators. */
+/* Running (time-averaged) statistics accumulated on the device in FP64: folds
+ * the current instantaneous profiles (as dnsg_statistics computes them) into the
+ * accumulators.  The averages are plane-and-time means; second moments combine
+ * the mean plane variance with the variance of the plane means. */
 int dnsg_stats_accumulate(dnsg_t *h);
 /* Returns the time averages: 9*NZ doubles as above (U, V, W, u_rms, v_rms, w_rms,
  * u'w', rho, rho_rms) plus *count samples. */
