@@ -310,6 +310,23 @@ static inline void collide(double *f, double Gx, double tau) {
     }
 }
 
+
+/* ---- single-node hooks for the closed-form tests ---------------------------------- */
+void orc_tables(double *wq, double *bi, int *e, int *opp, int *mir) {
+    int a;
+    for (a = 0; a < 19; a++) {
+        wq[a] = w[a]; bi[a] = BI[a];
+        e[a] = e0[a]; e[19 + a] = e1[a]; e[38 + a] = e2[a];
+        opp[a] = opposite[a]; mir[a] = mirror[a];
+    }
+}
+void orc_feq_node(double dens, const double *u, double *f) {
+    double v = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    int a;
+    for (a = 0; a < 19; a++) f[a] = feq_of(a, dens, u, v);
+}
+void orc_collide_node(double *f, double G, double omega) { collide(f, G, omega); }
+
 /* ---- initial states ----------------------------------------------------------- */
 void orc_init_rest(orc_t *o) { /* init.c:15-56; walls stay zero (calloc) */
     size_t n;

@@ -91,6 +91,12 @@ void orc_profile2d(const orc_t *o, double *jj, double *rho);       /* output_two
 void orc_get_levels(const orc_t *o, int upper, double *out);       /* [3][Xc][Yc][23], newest first */
 void orc_set_levels(orc_t *o, int upper, const double *in);
 
+/* single-node hooks (closed-form tests): lattice tables (e is [3][19]),
+ * equilibrium and the BGK+force collision of xcomputations.c:59-85 */
+void orc_tables(double *w, double *bi, int *e, int *opp, int *mir);
+void orc_feq_node(double dens, const double *u, double *f);
+void orc_collide_node(double *f, double G, double omega);
+
 /* reference file layouts */
 int orc_write_checkpoint(const orc_t *o, const char *dir, long index); /* main:1262-1341 */
 int orc_read_checkpoint(orc_t *o, const char *dir, long index);        /* main:999-1183 */

@@ -67,6 +67,11 @@ def oracle_lib():
         lib.orc_prng_bits.restype = ctypes.c_uint64
         lib.orc_profile_len.argtypes = [ctypes.c_void_p]
         lib.orc_profile2d.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        D = ctypes.POINTER(ctypes.c_double)
+        I = ctypes.POINTER(ctypes.c_int)
+        lib.orc_tables.argtypes = [D, D, I, I, I]
+        lib.orc_feq_node.argtypes = [ctypes.c_double, D, D]
+        lib.orc_collide_node.argtypes = [D, ctypes.c_double, ctypes.c_double]
         _olib = lib
     return _olib
 
