@@ -136,6 +136,8 @@ def main():
         res[prec] = run_gpu(mod, prec, args.steps, args.warmup)
         print(f"bench: {prec} {res[prec]['glups']:.3f} GLUPS, {res[prec]['ms_per_step']:.3f} ms/step, "
               f"kernels {res[prec]['kernels_ms']}", file=sys.stderr, flush=True)
+    probe = {k: round(mod.hbm_probe(4 << 30, i, 5), 1) for i, k in enumerate(("copy", "read", "write"))}
+    print(f"bench: HBM probe {probe} GB/s", file=sys.stderr, flush=True)
     main_t = res["fp64"] if "fp64" in res else res[tiers[0]]
     main_prec = "fp64" if "fp64" in res else tiers[0]
     bl = main_t["band_launch"]
@@ -167,6 +169,7 @@ def main():
         "hbm_roofline_frac_step": round(main_t["glups"] * BYTES_PER_LU[main_prec] / HBM_PEAK_GBS, 4),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "hbm_probe_gbs": probe,
         "tiers": {k: {"glups": round(v["glups"], 4), "ms_per_step": round(v["ms_per_step"], 4),
                       "hbm_frac": round(v["glups"] * BYTES_PER_LU[k] / HBM_PEAK_GBS, 4),
                       "kernels_ms": {a: round(b, 4) for a, b in v["kernels_ms"].items()}}

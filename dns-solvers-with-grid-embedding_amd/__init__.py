@@ -79,6 +79,7 @@ SIGNATURES = {
     "dnsg_step": [_P, ctypes.c_int, ctypes.POINTER(Diag)],
     "dnsg_run_timed": [_P, ctypes.c_int, ctypes.POINTER(Diag), _DP],
     "dnsg_kernel_times": [_P, ctypes.c_int, _DP],
+    "dnsg_hbm_probe": [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, _DP],
     "dnsg_encalc2": [_P, ctypes.c_long, ctypes.POINTER(Diag)],
     "dnsg_wshearstr": [_P, ctypes.c_long, ctypes.POINTER(Diag)],
     "dnsg_xcomputations": [_P],
@@ -280,6 +281,15 @@ class Solver:
 
     def fexchange(self):
         self._check(self.lib.dnsg_fexchange(self.h))
+
+
+def hbm_probe(nbytes=2 << 30, kind=0, reps=5):
+    """achievable HBM bandwidth (GB/s): kind 0 copy, 1 read, 2 write"""
+    g = ctypes.c_double()
+    rc = load().dnsg_hbm_probe(nbytes, kind, reps, ctypes.byref(g))
+    if rc:
+        raise DnsgError(rc, "hbm probe")
+    return g.value
 
 
 def diag_array(d, n):

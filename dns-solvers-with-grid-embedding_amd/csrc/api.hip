@@ -143,6 +143,9 @@ LatArgs coarse_args(dnsg_t *h) {
     L.wlo = nullptr; L.whi = nullptr;
     L.gscale = 1.;
     L.omega = h->d.omega_c;
+    L.skip_lo = 1;  // coarse k = 0, zdc-1 are rewritten by the fine->coarse injection
+    L.skip_hi = 1;
+    L.red = nullptr;
     return L;
 }
 
@@ -159,11 +162,20 @@ void band_args(dnsg_t *h, LatArgs *L) {
         L[b].whi = b == 0 ? nullptr : h->wall_up;
         L[b].gscale = gs;
         L[b].omega = h->d.omega_f;
+        // the interface layer (lower: top stored layer, upper: layer 0) is rewritten
+        // by the coarse->fine transfer after every substep
+        L[b].skip_lo = b == 0 ? 0 : 1;
+        L[b].skip_hi = b == 0 ? 1 : 0;
+        L[b].red = nullptr;
     }
 }
 
+double *red_coarse(dnsg_t *h) { return h->red; }
+double *red_lower(dnsg_t *h) { return h->red + (size_t)h->coarse.ny * h->coarse.nz * RED_W; }
+double *red_upper(dnsg_t *h) { return red_lower(h) + (size_t)h->lower.ny * h->lower.nz * RED_W; }
+
 void rotate_levels(dnsg_t *h) {  // xcomputations.c:548-556
-    double *t = h->lev_lo[2];
+    void *t = h->lev_lo[2];
     h->lev_lo[2] = h->lev_lo[1]; h->lev_lo[1] = h->lev_lo[0]; h->lev_lo[0] = t;
     t = h->lev_up[2];
     h->lev_up[2] = h->lev_up[1]; h->lev_up[1] = h->lev_up[0]; h->lev_up[0] = t;
@@ -199,15 +211,22 @@ struct SecTimer {
     }
 };
 
-// one coarse step of main:1256-1375; rec receives the diag record on the device
+// One coarse step of main:1256-1375; rec receives the diag record on the device.
+// The reductions of encalc2 and wshearstr read the state at the top of the step,
+// which is exactly what the coarse collide-stream and the first band substep
+// load: they sum it there (per-row partials) instead of re-reading the lattices.
+// Only the flux controller needs G = ub - <rho u> before the coarse collision, so
+// in that mode encalc2 keeps its own reduction pass.
 int do_step(dnsg_t *h, double *rec, SecTimer *T = nullptr) {
-    int rc;
+    const bool flux = h->p.force_mode == DNSG_FORCE_FLUX;
     if (T) T->mark(h->stream, -1);
-    HIPCHK(h, launch_encalc2(h->stream, h, rec));
-    HIPCHK(h, launch_wshear(h->stream, h, rec));
-    if (T) T->mark(h->stream, SEC_REDUCE);
+    if (flux) {
+        HIPCHK(h, launch_encalc2(h->stream, h, rec));
+        if (T) T->mark(h->stream, SEC_REDUCE);
+    }
     {
         LatArgs L = coarse_args(h);
+        if (!flux) L.red = red_coarse(h);
         HIPCHK(h, launch_collide_stream(h->stream, &L, 1, h->d_force, h->p.precision));
         if (T) T->mark(h->stream, SEC_COARSE);
         rotate_levels(h);
@@ -215,8 +234,19 @@ int do_step(dnsg_t *h, double *rec, SecTimer *T = nullptr) {
         if (T) T->mark(h->stream, SEC_SAMPLE);
     }
     for (int iter = 1; iter <= h->d.GR; iter++) {
-        if ((rc = do_xfcomputations(h))) return rc;
+        LatArgs L[2];
+        band_args(h, L);
+        if (iter == 1) {
+            L[0].red = red_lower(h);
+            L[1].red = red_upper(h);
+        }
+        HIPCHK(h, launch_collide_stream(h->stream, L, 2, h->d_force, h->p.precision));
         if (T) T->mark(h->stream, SEC_BAND);
+        if (iter == 1) {
+            if (!flux) HIPCHK(h, launch_encalc2_fused(h->stream, h, rec));
+            HIPCHK(h, launch_wshear_fused(h->stream, h, rec));
+            if (T) T->mark(h->stream, SEC_REDUCE);
+        }
         HIPCHK(h, launch_interface(h->stream, h, iter));
         if (iter == h->d.GR) HIPCHK(h, launch_ftcd(h->stream, h));
         if (T) T->mark(h->stream, SEC_IFACE);
@@ -240,35 +270,48 @@ void fill_diag(const dnsg_t *h, const double *rec, long ss, dnsg_diag *d) {
     d->pgrad = rec[8];
 }
 
-// [3][Xc][Yc][23] (file layout, newest first) <-> three device planes [23][Yc][Xc]
-int put_levels(dnsg_t *h, const double *lv, double **dev) {
+// [3][Xc][Yc][23] (file layout, newest first) <-> three device planes [23][Yc][Xc];
+// the shifted tiers keep FP32 planes with rho - 1 in slot 0
+int put_levels(dnsg_t *h, const double *lv, void *const *dev) {
     const int Xc = h->d.Xc, Yc = h->d.Yc;
     const size_t np = (size_t)Xc * Yc * NVP;
+    const bool f32 = h->p.precision != DNSG_PREC_FP64;
     std::vector<double> t(np);
+    std::vector<float> tf(f32 ? np : 0);
     for (int l = 0; l < 3; l++) {
         const double *src = lv + l * np;
         for (int i = 0; i < Xc; i++)
             for (int j = 0; j < Yc; j++)
-                for (int v = 0; v < NVP; v++)
-                    t[((size_t)v * Yc + j) * Xc + i] = src[((size_t)i * Yc + j) * NVP + v];
-        HIPCHK(h, hipMemcpyAsync(dev[l], t.data(), np * 8, hipMemcpyHostToDevice, h->stream));
+                for (int v = 0; v < NVP; v++) {
+                    const double x = src[((size_t)i * Yc + j) * NVP + v];
+                    const size_t d = ((size_t)v * Yc + j) * Xc + i;
+                    if (f32) tf[d] = (float)(v == 0 ? x - 1. : x);
+                    else t[d] = x;
+                }
+        if (f32) HIPCHK(h, hipMemcpyAsync(dev[l], tf.data(), np * 4, hipMemcpyHostToDevice, h->stream));
+        else HIPCHK(h, hipMemcpyAsync(dev[l], t.data(), np * 8, hipMemcpyHostToDevice, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
     }
     return DNSG_OK;
 }
 
-int get_levels(dnsg_t *h, double *lv, double *const *dev) {
+int get_levels(dnsg_t *h, double *lv, void *const *dev) {
     const int Xc = h->d.Xc, Yc = h->d.Yc;
     const size_t np = (size_t)Xc * Yc * NVP;
+    const bool f32 = h->p.precision != DNSG_PREC_FP64;
     std::vector<double> t(np);
+    std::vector<float> tf(f32 ? np : 0);
     for (int l = 0; l < 3; l++) {
-        HIPCHK(h, hipMemcpyAsync(t.data(), dev[l], np * 8, hipMemcpyDeviceToHost, h->stream));
+        if (f32) HIPCHK(h, hipMemcpyAsync(tf.data(), dev[l], np * 4, hipMemcpyDeviceToHost, h->stream));
+        else HIPCHK(h, hipMemcpyAsync(t.data(), dev[l], np * 8, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
         double *dst = lv + l * np;
         for (int i = 0; i < Xc; i++)
             for (int j = 0; j < Yc; j++)
-                for (int v = 0; v < NVP; v++)
-                    dst[((size_t)i * Yc + j) * NVP + v] = t[((size_t)v * Yc + j) * Xc + i];
+                for (int v = 0; v < NVP; v++) {
+                    const size_t d = ((size_t)v * Yc + j) * Xc + i;
+                    dst[((size_t)i * Yc + j) * NVP + v] = f32 ? (v == 0 ? 1. + (double)tf[d] : (double)tf[d]) : t[d];
+                }
     }
     return DNSG_OK;
 }
@@ -382,7 +425,7 @@ int dnsg_destroy(dnsg_t *h) {
         if (h->lev_lo[l]) (void)hipFree(h->lev_lo[l]);
         if (h->lev_up[l]) (void)hipFree(h->lev_up[l]);
     }
-    void *ptrs[] = {h->slots_lo, h->slots_up, h->wall_lo, h->wall_up, h->part, h->d_force, h->d_diag, h->d_red, h->d_acc};
+    void *ptrs[] = {h->wall_lo, h->wall_up, h->part, h->red, h->d_force, h->d_diag, h->d_red, h->d_acc};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (h->h_pinned) (void)hipHostFree(h->h_pinned);
@@ -426,8 +469,6 @@ int dnsg_create(const dnsg_params *p, dnsg_t **out) {
         HIPCHK(h, hipMemsetAsync(h->lev_lo[l], 0, np * 8, h->stream));
         HIPCHK(h, hipMemsetAsync(h->lev_up[l], 0, np * 8, h->stream));
     }
-    HIPCHK(h, hipMalloc(&h->slots_lo, 4 * np * 8));
-    HIPCHK(h, hipMalloc(&h->slots_up, 4 * np * 8));
     const std::vector<int8_t> w = wall_codes(p, &d);
     HIPCHK(h, hipMalloc(&h->wall_lo, w.size()));
     HIPCHK(h, hipMalloc(&h->wall_up, w.size()));
@@ -435,6 +476,11 @@ int dnsg_create(const dnsg_params *p, dnsg_t **out) {
     HIPCHK(h, hipMemcpy(h->wall_up, w.data(), w.size(), hipMemcpyHostToDevice));
     h->part_cap = partial_doubles(h);
     HIPCHK(h, hipMalloc(&h->part, h->part_cap * 8));
+    {
+        const size_t nred = ((size_t)d.Yc * d.zdc + 2 * (size_t)d.Yf * (d.zdf - 1)) * RED_W;
+        HIPCHK(h, hipMalloc(&h->red, nred * 8));
+        HIPCHK(h, hipMemsetAsync(h->red, 0, nred * 8, h->stream));
+    }
     HIPCHK(h, hipMalloc(&h->d_force, 4 * 8));
     HIPCHK(h, hipMemsetAsync(h->d_force, 0, 4 * 8, h->stream));
     const size_t nred = 4 * ((size_t)d.zdc + 2 * (d.zdf - 1)) + 2 * 9 * (size_t)d.NZ + 64;

@@ -1,16 +1,18 @@
 // internal.hpp -- device-side state of one solver handle and the kernel launchers.
 //
-// Layout in HBM (structure of arrays, x fastest, one plane per direction):
+// Layout in HBM (row-interleaved structure of arrays, x fastest): the 19
+// direction vectors of one (z, y) row are stored back to back,
 //   lattice g, direction q, stored layer z, row y, column x ->
-//       base[g] + q*qs + (z*ny + y)*nx + x,   qs = nx*ny*nz
+//       base[g] + ((z*ny + y)*19 + q)*nx + x
+// so one workgroup's reads are a single contiguous 19*nx block and its writes
+// fall in the 9 neighbouring rows' blocks.
 // The coarse grid stores all zdc layers.  A band stores its zds fluid layers
 // only (the wall layer is never collided or streamed and is kept on the host):
 //   lower band: stored layer m <-> reference k = m+1  (wall below m = 0,
 //               interface layer m = zds-1)
 //   upper band: stored layer m <-> reference k = m    (interface layer m = 0,
 //               wall above m = zds-1)
-// Interface planes are [23][Yc][Xc]; the interpolation slots are
-// [4][23][Yc][Xc] (value, d/dx, d/dy, d2/dxdy).
+// Interface planes are [23][Yc][Xc] (rho, u, f_neq[19]).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stddef.h>
@@ -25,6 +27,11 @@ namespace dnsg {
 enum BoundKind { BOUND_DROP = 0, BOUND_WALL = 1 };
 
 // one lattice as the streaming kernels see it
+// element (q, row = z*ny + y, x) of a lattice with row length nx
+__host__ __device__ __forceinline__ size_t lat_at(int q, size_t row, int x, int nx) {
+    return (row * 19 + (size_t)q) * (size_t)nx + (size_t)x;
+}
+
 struct LatArgs {
     const void *cur;   // pre-collision state (read)
     void *nxt;         // post-stream state (written)
@@ -34,7 +41,13 @@ struct LatArgs {
     const int8_t *whi;     // ... and above
     double gscale;         // G = gscale * force (1 for coarse, 1/GR for the bands)
     double omega;          // 1/tau
+    int skip_lo, skip_hi;  // do not write into layer 0 / nz-1 (rewritten by the interface)
+    double *red;           // if set: per-row sums of the pre-collision state, RED_W per row
 };
+
+// per-row sums written by the collide-stream kernels when LatArgs::red is set:
+// sum rho, sum rho u_x, sum u_x (u_x = rho u_x / rho)
+constexpr int RED_W = 3;
 
 struct Grid {
     void *buf[2] = {nullptr, nullptr};  // A/B lattices
@@ -63,14 +76,15 @@ struct dnsg_ctx {
     size_t elem = 8;  // bytes per stored distribution value
 
     dnsg::Grid coarse, lower, upper;
-    // interface time levels, newest first; rotated by pointer swaps
-    double *lev_lo[3] = {nullptr, nullptr, nullptr};
-    double *lev_up[3] = {nullptr, nullptr, nullptr};
-    double *slots_lo = nullptr, *slots_up = nullptr;  // [4][23][Yc][Xc]
+    // interface time levels [23][Yc][Xc], newest first, rotated by pointer swaps;
+    // FP64 in the plain tier, FP32 (slot 0 = rho - 1) in the shifted tiers
+    void *lev_lo[3] = {nullptr, nullptr, nullptr};
+    void *lev_up[3] = {nullptr, nullptr, nullptr};
     int8_t *wall_lo = nullptr, *wall_up = nullptr;     // [Yf][Xf]
 
     // reductions
-    double *part = nullptr;     // row partials
+    double *part = nullptr;     // row partials of the explicit reduction passes
+    double *red = nullptr;      // row sums written by the collide-stream kernels (RED_W per row)
     size_t part_cap = 0;        // doubles
     double *d_force = nullptr;  // device copy of V.force[0]
     double *d_diag = nullptr;   // DIAG_WORDS per step
@@ -102,10 +116,12 @@ hipError_t launch_aos_to_soa(hipStream_t s, const double *aos, void *soa, int nx
                              int nz, int prec);
 hipError_t launch_soa_to_aos(hipStream_t s, const void *soa, double *aos, int nx, int ny, int nz_aos, int z0,
                              int nz, int prec);
-hipError_t launch_sample(hipStream_t s, dnsg_ctx *h, const void *coarse_state, double *lo, double *up);
+hipError_t launch_sample(hipStream_t s, dnsg_ctx *h, const void *coarse_state, void *lo, void *up);
 hipError_t launch_interface(hipStream_t s, dnsg_ctx *h, int iter);
 hipError_t launch_ftcd(hipStream_t s, dnsg_ctx *h);
 hipError_t launch_encalc2(hipStream_t s, dnsg_ctx *h, double *diag_rec);
+hipError_t launch_encalc2_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec);
+hipError_t launch_wshear_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec);
 hipError_t launch_wshear(hipStream_t s, dnsg_ctx *h, double *diag_rec);
 hipError_t launch_mass(hipStream_t s, dnsg_ctx *h, double *out);
 hipError_t launch_statistics(hipStream_t s, dnsg_ctx *h, double *out9nz);

@@ -3,19 +3,19 @@
 //
 //   sample     coarse k = 1 / zdc-2 after streaming -> (rho, u, f - feq) planes
 //              (xcomputations.c:558-625)
-//   hermite    cubic Hermite in time of the three newest planes
-//              (tinterpolate_hrmt.c:16-55)
-//   deriv      4th-order central differences d/dx, d/dy, d2/dxdy
-//              (derivative.c:17-89)
-//   ctfd       bicubic patch per coarse cell (bcucof_opt.c:16-62), evaluated at
-//              the GR x GR fine columns, f = feq + tau_f/(GR tau_c) f_neq on the
-//              band interface layer (ctfdtransfer_opt.c:17-175)
+//   iface      per substep and band: cubic Hermite in time of the three newest
+//              planes (tinterpolate_hrmt.c:16-55), 4th-order d/dx, d/dy,
+//              d2/dxdy (derivative.c:17-89), the bicubic patch of each coarse
+//              cell (bcucof_opt.c:16-62) evaluated at its GR x GR fine columns,
+//              and f = feq + tau_f/(GR tau_c) f_neq on the band interface layer
+//              (ctfdtransfer_opt.c:17-175) -- one LDS-tiled kernel
 //   ftcd       fine -> coarse injection at coincident nodes,
 //              f = feq + GR tau_c/tau_f f_neq (ftcdtransfer.c:16-118)
-// All planes are periodic in x and y, FP64, laid out [value][Yc][Xc].
-// In the FP64 tier every expression keeps the reference's operation order, so
-// the results are bitwise those of the reference.  In the shifted tiers the
-// lattices hold g = f - w; planes stay FP64 and f_neq = g - (feq - w).
+// Planes are periodic in x and y and laid out [value][Yc][Xc].
+// FP64 tier: FP64 planes, every expression in the reference's operation order,
+// so the results are bitwise those of the reference.
+// Shifted tiers (lattices hold g = f - w): FP32 planes and arithmetic, slot 0
+// holds rho - 1 instead of rho, and f_neq = g - (feq - w).
 #include "internal.hpp"
 #include "lattice.hpp"
 
@@ -24,20 +24,20 @@ namespace dnsg {
 constexpr int NV = 23;  // rho, u_x, u_y, u_z, f_neq[19]
 
 // ---- sampling --------------------------------------------------------------------
-template <class S>
-__global__ __launch_bounds__(256) void k_sample(const S *__restrict__ cs, int nx, int ny, int nz, int k_lo, int k_up,
-                                                double *__restrict__ plo, double *__restrict__ pup) {
+template <class S, class P>
+__global__ __launch_bounds__(256) void k_sample(const S *__restrict__ cs, int nx, int ny, int k_lo, int k_up,
+                                                P *__restrict__ plo, P *__restrict__ pup) {
     const int n = nx * ny;
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= n) return;
     const int k = blockIdx.y == 0 ? k_lo : k_up;
-    double *pl = blockIdx.y == 0 ? plo : pup;
-    const size_t qs = (size_t)nx * ny * nz, base = (size_t)k * n + id;
-    double f[Q];
+    P *pl = blockIdx.y == 0 ? plo : pup;
+    const size_t row = (size_t)k * ny + id / nx;
+    const int x = id % nx;
+    P f[Q];
     unroll<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        if constexpr (sizeof(S) == 8) f[q] = cs[q * qs + base];
-        else f[q] = (double)load_s(cs + q * qs + base);
+        f[q] = (P)load_s(cs + lat_at(q, row, x, nx));
     });
     if constexpr (sizeof(S) == 8) {
         const Macro64 m = macro64(f);
@@ -50,92 +50,46 @@ __global__ __launch_bounds__(256) void k_sample(const S *__restrict__ cs, int nx
             pl[(4 + q) * n + id] = f[q] - feq64<q>(m.dens, m.u0, m.u1, m.u2, m.v);
         });
     } else {
-        const double drho = zeroth_moment(f), dens = 1. + drho;
-        const double u0 = first_moment<0>(f) / dens, u1 = first_moment<1>(f) / dens, u2 = first_moment<2>(f) / dens;
-        const double v = u0 * u0 + u1 * u1 + u2 * u2;
-        pl[0 * n + id] = dens;
-        pl[1 * n + id] = u0;
-        pl[2 * n + id] = u1;
-        pl[3 * n + id] = u2;
+        const Macro32 m = macro32(f);
+        pl[0 * n + id] = m.drho;
+        pl[1 * n + id] = m.u0;
+        pl[2 * n + id] = m.u1;
+        pl[3 * n + id] = m.u2;
         unroll<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const double eu = edot<q>(u0, u1, u2);
-            const double geq = WQ[q] * (drho + dens * (3. * eu + 4.5 * eu * eu - 1.5 * v));
-            pl[(4 + q) * n + id] = f[q] - geq;
+            pl[(4 + q) * n + id] = f[q] - geq32<q>(m);
         });
     }
 }
 
-hipError_t launch_sample(hipStream_t s, dnsg_ctx *h, const void *coarse_state, double *lo, double *up) {
+hipError_t launch_sample(hipStream_t s, dnsg_ctx *h, const void *coarse_state, void *lo, void *up) {
     const Grid &g = h->coarse;
     const int n = g.nx * g.ny;
     dim3 grd((n + 255) / 256, 2), blk(256);
     const int prec = h->p.precision;
     if (prec == DNSG_PREC_FP64)
-        hipLaunchKernelGGL(k_sample<double>, grd, blk, 0, s, (const double *)coarse_state, g.nx, g.ny, g.nz, 1,
-                           g.nz - 2, lo, up);
+        hipLaunchKernelGGL((k_sample<double, double>), grd, blk, 0, s, (const double *)coarse_state, g.nx, g.ny, 1,
+                           g.nz - 2, (double *)lo, (double *)up);
     else if (prec == DNSG_PREC_FP32)
-        hipLaunchKernelGGL(k_sample<float>, grd, blk, 0, s, (const float *)coarse_state, g.nx, g.ny, g.nz, 1,
-                           g.nz - 2, lo, up);
+        hipLaunchKernelGGL((k_sample<float, float>), grd, blk, 0, s, (const float *)coarse_state, g.nx, g.ny, 1,
+                           g.nz - 2, (float *)lo, (float *)up);
     else
-        hipLaunchKernelGGL(k_sample<__half>, grd, blk, 0, s, (const __half *)coarse_state, g.nx, g.ny, g.nz, 1,
-                           g.nz - 2, lo, up);
+        hipLaunchKernelGGL((k_sample<__half, float>), grd, blk, 0, s, (const __half *)coarse_state, g.nx, g.ny, 1,
+                           g.nz - 2, (float *)lo, (float *)up);
     return hipGetLastError();
 }
 
-// ---- Hermite in time -----------------------------------------------------------------
+// ---- helpers ---------------------------------------------------------------------------
+template <class P>
 struct HermiteW {
-    double h00, h10, h01, h11;
+    P h00, h10, h01, h11;
 };
 
-struct Levels {
-    const double *ln, *lp, *lpp;
-    double *out;
-};
-
-__global__ __launch_bounds__(256) void k_hermite(Levels L0, Levels L1, int n, HermiteW H) {
-    const Levels &L = blockIdx.y == 0 ? L0 : L1;
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n) return;
-    const double ln = L.ln[id], lp = L.lp[id], lpp = L.lpp[id];
-    const double c1 = lp;
-    const double c2 = (ln - lpp) * 0.5;
-    const double c3 = ln;
-    const double c4 = (3. * ln - 4. * lp + lpp) * 0.5;
-    L.out[id] = H.h00 * c1 + H.h10 * c2 + H.h01 * c3 + H.h11 * c4;
+template <class P>
+__device__ __forceinline__ P d4(P fm2, P fm1, P fp1, P fp2) {
+    return (P)(2. / 3.) * (fp1 - fm1) - (P)(1. / 12.) * (fp2 - fm2);
 }
 
-// ---- 4th-order derivatives (periodic) -------------------------------------------------
-__device__ __forceinline__ double d4(double fm2, double fm1, double fp1, double fp2) {
-    return (2. / 3.) * (fp1 - fm1) - (1. / 12.) * (fp2 - fm2);
-}
-
-__global__ __launch_bounds__(256) void k_deriv12(double *s_lo, double *s_up, int nx, int ny) {
-    double *sl = blockIdx.y == 0 ? s_lo : s_up;
-    const int n = nx * ny;
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n * NV) return;
-    const int v = id / n, r = id % n, x = r % nx, y = r / nx;
-    const double *s0 = sl + (size_t)v * n;
-    auto X = [&](int d) { int t = x + d; t += (t < 0) ? nx : 0; t -= (t >= nx) ? nx : 0; return t; };
-    auto Y = [&](int d) { int t = y + d; t += (t < 0) ? ny : 0; t -= (t >= ny) ? ny : 0; return t; };
-    sl[(size_t)(1 * NV + v) * n + r] = d4(s0[y * nx + X(-2)], s0[y * nx + X(-1)], s0[y * nx + X(1)], s0[y * nx + X(2)]);
-    sl[(size_t)(2 * NV + v) * n + r] =
-        d4(s0[Y(-2) * nx + x], s0[Y(-1) * nx + x], s0[Y(1) * nx + x], s0[Y(2) * nx + x]);
-}
-
-__global__ __launch_bounds__(256) void k_deriv3(double *s_lo, double *s_up, int nx, int ny) {
-    double *sl = blockIdx.y == 0 ? s_lo : s_up;
-    const int n = nx * ny;
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n * NV) return;
-    const int v = id / n, r = id % n, x = r % nx, y = r / nx;
-    const double *s2 = sl + (size_t)(2 * NV + v) * n;
-    auto X = [&](int d) { int t = x + d; t += (t < 0) ? nx : 0; t -= (t >= nx) ? nx : 0; return t; };
-    sl[(size_t)(3 * NV + v) * n + r] = d4(s2[y * nx + X(-2)], s2[y * nx + X(-1)], s2[y * nx + X(1)], s2[y * nx + X(2)]);
-}
-
-// ---- bicubic coefficients and the fine interface layer ------------------------------
 // the bicubic interpolation matrix of bcucof_opt.c:18-35 (Press et al., bcucof)
 constexpr int WT[16][16] = {
     {1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
@@ -156,133 +110,228 @@ constexpr int WT[16][16] = {
     {4, -4, 4, -4, 2, 2, -2, -2, 2, -2, -2, 2, 1, 1, 1, 1}};
 
 // c = WT x, summed in k order from 0 with the zero weights dropped
-__device__ __forceinline__ void bicubic_coef(const double *xv, double *c) {
+template <class P>
+__device__ __forceinline__ void bicubic_coef(const P *xv, P *c) {
     unroll<0, 16>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
-        double xx = 0.0;
+        P xx = P(0);
         unroll<0, 16>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            if constexpr (WT[i][k] != 0) xx = xx + (double)WT[i][k] * xv[k];
+            if constexpr (WT[i][k] != 0) xx = xx + (P)WT[i][k] * xv[k];
         });
         c[i] = xx;
     });
 }
 
 // Horner form of ctfdtransfer_opt.c:59-62
-__device__ __forceinline__ double bicubic_eval(const double *c, double tt, double uu) {
-    double r = 0.;
+template <class P>
+__device__ __forceinline__ P bicubic_eval(const P *c, P tt, P uu) {
+    P r = P(0);
     for (int nn = 3; nn > -1; nn--) r = tt * r + ((c[nn * 4 + 3] * uu + c[nn * 4 + 2]) * uu + c[nn * 4 + 1]) * uu + c[nn * 4 + 0];
     return r;
 }
 
-__device__ __forceinline__ void gather16(const double *sl, int n, int v, int p00, int p10, int p11, int p01,
-                                         double *xv) {
-    const double *s0 = sl + (size_t)(0 * NV + v) * n, *s1 = sl + (size_t)(1 * NV + v) * n;
-    const double *s2 = sl + (size_t)(2 * NV + v) * n, *s3 = sl + (size_t)(3 * NV + v) * n;
-    xv[0] = s0[p00]; xv[4] = s1[p00]; xv[8] = s2[p00];  xv[12] = s3[p00];
-    xv[1] = s0[p10]; xv[5] = s1[p10]; xv[9] = s2[p10];  xv[13] = s3[p10];
-    xv[2] = s0[p11]; xv[6] = s1[p11]; xv[10] = s2[p11]; xv[14] = s3[p11];
-    xv[3] = s0[p01]; xv[7] = s1[p01]; xv[11] = s2[p01]; xv[15] = s3[p01];
+__constant__ int c_EX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, 1, -1, -1, 0, 0, 0, 0};
+__constant__ int c_EY[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, 1, -1, -1};
+__constant__ int c_EZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+__constant__ double c_WQ[19] = {1. / 3., 1. / 18., 1. / 18., 1. / 18., 1. / 18., 1. / 18., 1. / 18.,
+                                1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36.,
+                                1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36.};
+
+// e_q . u for a runtime direction, with the same operation order as edot<q>
+template <class P>
+__device__ __forceinline__ P edot_rt(int q, P u0, P u1, P u2) {
+    P s = P(0);
+    bool have = false;
+    const int e[3] = {c_EX[q], c_EY[q], c_EZ[q]};
+    const P u[3] = {u0, u1, u2};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if (e[c] == 0) continue;
+        const P t = e[c] > 0 ? u[c] : -u[c];
+        s = have ? s + t : t;
+        have = true;
+    }
+    return s;
 }
 
-struct CtfdBand {
-    const double *slots;
-    void *band;   // band post-stream lattice
-    int layer;    // stored interface layer
+// ---- fused interface fill ----------------------------------------------------------
+// One workgroup = one TX x TY tile of coarse cells of one band.  For each of the
+// 23 plane values in turn it
+//   1. forms the Hermite-in-time value on the tile plus a 2-cell halo, in LDS
+//      (the loads of the next value are issued before this value is processed);
+//   2. takes d/dx, d/dy at the tile's (TX+1) x (TY+1) corners (d/dy also on the
+//      halo columns) and d2/dxdy = d/dx of d/dy, in LDS;
+//   3. builds each cell's 16 bicubic coefficients from its 4 corners and
+//      evaluates them at the cell's GR x GR fine columns;
+//   4. keeps rho and u in registers; for a distribution value writes
+//      f = feq + tau_f/(GR tau_c) f_neq through an LDS tile so that the
+//      interface-layer rows are stored contiguously.
+struct IfaceBand {
+    const void *ln, *lp, *lpp;  // newest, previous, oldest plane levels [23][Yc][Xc]
+    void *band;                 // band post-stream lattice
+    int layer;                  // stored interface layer
 };
 
-template <class S, int GR>
-__global__ __launch_bounds__(128) void k_ctfd(CtfdBand B0, CtfdBand B1, int Xc, int Yc, int nzb, double rescale) {
-    const CtfdBand &B = blockIdx.y == 0 ? B0 : B1;
-    const int n = Xc * Yc;
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n) return;
-    const int ic = id % Xc, jc = id / Xc;
-    const int in = (ic + 1 == Xc) ? 0 : ic + 1, jn = (jc + 1 == Yc) ? 0 : jc + 1;
-    const int p00 = jc * Xc + ic, p10 = jc * Xc + in, p11 = jn * Xc + in, p01 = jn * Xc + ic;
-    const int Xf = GR * Xc, Yf = GR * Yc;
-    const size_t qs = (size_t)Xf * Yf * nzb;
+template <class S, class P, int GR, int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void k_iface(IfaceBand B0, IfaceBand B1, int Xc, int Yc, HermiteW<P> H,
+                                                  P rescale) {
+    constexpr int NT = TX * TY, HX = TX + 5, HY = TY + 5, NP = GR * GR, FX = GR * TX, FY = GR * TY;
+    constexpr int NH = (HX * HY + NT - 1) / NT;  // halo points per thread
+    __shared__ P s0[HY][HX];          // value, halo 2 on each side (+1 corner)
+    __shared__ P s1[TY + 1][TX + 1];  // d/dx at the corners
+    __shared__ P s2[TY + 1][HX];      // d/dy at the corners and the halo columns
+    __shared__ P s3[TY + 1][TX + 1];  // d2/dxdy at the corners
+    __shared__ P fo[FY][FX];          // output tile of one direction
+    const IfaceBand &B = blockIdx.y == 0 ? B0 : B1;
+    const P *__restrict__ Ln = static_cast<const P *>(B.ln);
+    const P *__restrict__ Lp = static_cast<const P *>(B.lp);
+    const P *__restrict__ Lpp = static_cast<const P *>(B.lpp);
+    const int ntx = (Xc + TX - 1) / TX;
+    const int x0 = (blockIdx.x % ntx) * TX, y0 = (blockIdx.x / ntx) * TY;
+    const int tid = threadIdx.x, cx = tid % TX, cy = tid / TX;
+    const int n = Xc * Yc, Xf = GR * Xc, Yf = GR * Yc;
     S *band = static_cast<S *>(B.band);
-    const size_t lay0 = (size_t)B.layer * Yf * Xf;
-    constexpr int NP = GR * GR;
-    double dens[NP], u0[NP], u1[NP], u2[NP], vv[NP];
-    double xv[16], c[16];
-    // rho and u at the fine points
-    for (int v = 0; v < 4; v++) {
-        gather16(B.slots, n, v, p00, p10, p11, p01, xv);
-        bicubic_coef(xv, c);
+    // halo points of this thread and their plane offsets (fixed for every value)
+    int hoff[NH];
+    bool hok[NH];
 #pragma unroll
-        for (int p = 0; p < NP; p++) {
-            const int ii = p / GR, jj = p % GR;
-            const double tt = ((double)ii) / ((double)GR), uu = ((double)jj) / ((double)GR);
-            const double r = bicubic_eval(c, tt, uu);
-            if (v == 0) dens[p] = r;
-            else if (v == 1) u0[p] = r;
-            else if (v == 2) u1[p] = r;
-            else u2[p] = r;
-        }
+    for (int k = 0; k < NH; k++) {
+        const int p = tid + k * NT;
+        hok[k] = p < HX * HY;
+        const int gx = ((x0 - 2 + p % HX) % Xc + Xc) % Xc, gy = ((y0 - 2 + p / HX) % Yc + Yc) % Yc;
+        hoff[k] = hok[k] ? gy * Xc + gx : 0;
     }
+    P a[NH][3];
 #pragma unroll
-    for (int p = 0; p < NP; p++) vv[p] = u0[p] * u0[p] + u1[p] * u1[p] + u2[p] * u2[p];
-    unroll<0, Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        gather16(B.slots, n, 4 + q, p00, p10, p11, p01, xv);
-        bicubic_coef(xv, c);
+    for (int k = 0; k < NH; k++)
+        if (hok[k]) { a[k][0] = Ln[hoff[k]]; a[k][1] = Lp[hoff[k]]; a[k][2] = Lpp[hoff[k]]; }
+    P dens[NP], u0[NP], u1[NP], u2[NP], vv[NP];
+    for (int v = 0; v < NV; v++) {
+        // 1. Hermite value on the halo tile (tinterpolate_hrmt.c:28-34)
 #pragma unroll
-        for (int p = 0; p < NP; p++) {
-            const int ii = p / GR, jj = p % GR;
-            const double tt = ((double)ii) / ((double)GR), uu = ((double)jj) / ((double)GR);
-            const double fneq = bicubic_eval(c, tt, uu);
-            const size_t dst = q * qs + lay0 + (size_t)(GR * jc + jj) * Xf + (GR * ic + ii);
-            if constexpr (sizeof(S) == 8) {
-                const double feq = feq64<q>(dens[p], u0[p], u1[p], u2[p], vv[p]);
-                band[dst] = feq + rescale * fneq;
-            } else {
-                const double eu = edot<q>(u0[p], u1[p], u2[p]);
-                const double geq = WQ[q] * ((dens[p] - 1.) + dens[p] * (3. * eu + 4.5 * eu * eu - 1.5 * vv[p]));
-                store_s(band + dst, (float)(geq + rescale * fneq));
+        for (int k = 0; k < NH; k++) {
+            if (!hok[k]) continue;
+            const int p = tid + k * NT;
+            const P ln = a[k][0], lp = a[k][1], lpp = a[k][2];
+            const P c1 = lp;
+            const P c2 = (ln - lpp) * (P)0.5;
+            const P c3 = ln;
+            const P c4 = ((P)3. * ln - (P)4. * lp + lpp) * (P)0.5;
+            s0[p / HX][p % HX] = H.h00 * c1 + H.h10 * c2 + H.h01 * c3 + H.h11 * c4;
+        }
+        if (v + 1 < NV) {  // prefetch the next value
+            const size_t vo = (size_t)(v + 1) * n;
+#pragma unroll
+            for (int k = 0; k < NH; k++)
+                if (hok[k]) { a[k][0] = Ln[vo + hoff[k]]; a[k][1] = Lp[vo + hoff[k]]; a[k][2] = Lpp[vo + hoff[k]]; }
+        }
+        __syncthreads();
+        // 2. d/dy on corner rows for every halo column; d/dx at the corners (derivative.c:118-119)
+        for (int p = tid; p < (TY + 1) * HX; p += NT) {
+            const int r = p / HX, hx = p % HX, hy = r + 2;
+            s2[r][hx] = d4(s0[hy - 2][hx], s0[hy - 1][hx], s0[hy + 1][hx], s0[hy + 2][hx]);
+            if (hx >= 2 && hx <= TX + 2)
+                s1[r][hx - 2] = d4(s0[hy][hx - 2], s0[hy][hx - 1], s0[hy][hx + 1], s0[hy][hx + 2]);
+        }
+        __syncthreads();
+        // 3. d2/dxdy = d/dx of d/dy (derivative.c:139)
+        for (int p = tid; p < (TY + 1) * (TX + 1); p += NT) {
+            const int r = p / (TX + 1), c = p % (TX + 1), hx = c + 2;
+            s3[r][c] = d4(s2[r][hx - 2], s2[r][hx - 1], s2[r][hx + 1], s2[r][hx + 2]);
+        }
+        __syncthreads();
+        // 4. bicubic patch of this cell (bcucof_opt.c:47-61) at its fine columns
+        P r[NP];
+        {
+            P xv[16], c[16];
+            xv[0] = s0[cy + 2][cx + 2]; xv[4] = s1[cy][cx];         xv[8] = s2[cy][cx + 2];      xv[12] = s3[cy][cx];
+            xv[1] = s0[cy + 2][cx + 3]; xv[5] = s1[cy][cx + 1];     xv[9] = s2[cy][cx + 3];      xv[13] = s3[cy][cx + 1];
+            xv[2] = s0[cy + 3][cx + 3]; xv[6] = s1[cy + 1][cx + 1]; xv[10] = s2[cy + 1][cx + 3]; xv[14] = s3[cy + 1][cx + 1];
+            xv[3] = s0[cy + 3][cx + 2]; xv[7] = s1[cy + 1][cx];     xv[11] = s2[cy + 1][cx + 2]; xv[15] = s3[cy + 1][cx];
+            bicubic_coef(xv, c);
+#pragma unroll
+            for (int pp = 0; pp < NP; pp++) {
+                const int ii = pp / GR, jj = pp % GR;
+                r[pp] = bicubic_eval(c, (P)(((double)ii) / ((double)GR)), (P)(((double)jj) / ((double)GR)));
             }
         }
-    });
+        if (v < 4) {
+#pragma unroll
+            for (int pp = 0; pp < NP; pp++) {
+                if (v == 0) dens[pp] = r[pp];
+                else if (v == 1) u0[pp] = r[pp];
+                else if (v == 2) u1[pp] = r[pp];
+                else u2[pp] = r[pp];
+            }
+            if (v == 3) {
+#pragma unroll
+                for (int pp = 0; pp < NP; pp++) vv[pp] = u0[pp] * u0[pp] + u1[pp] * u1[pp] + u2[pp] * u2[pp];
+            }
+            __syncthreads();  // s0..s3 are rewritten by the next value
+            continue;
+        }
+        const int q = v - 4;
+        const P wq = (P)c_WQ[q];
+#pragma unroll
+        for (int pp = 0; pp < NP; pp++) {
+            const int ii = pp / GR, jj = pp % GR;
+            const P eu = edot_rt(q, u0[pp], u1[pp], u2[pp]);
+            P out;
+            if constexpr (sizeof(S) == 8) {
+                const P feq = wq * dens[pp] * ((P)1. + (P)3. * eu + (P)4.5 * eu * eu - (P)1.5 * vv[pp]);
+                out = feq + rescale * r[pp];  // ctfdtransfer_opt.c:93
+            } else {  // dens holds rho - 1
+                const P rho = (P)1. + dens[pp];
+                const P geq = wq * (dens[pp] + rho * ((P)3. * eu + (P)4.5 * eu * eu - (P)1.5 * vv[pp]));
+                out = geq + rescale * r[pp];
+            }
+            fo[GR * cy + jj][GR * cx + ii] = out;
+        }
+        __syncthreads();
+        for (int p = tid; p < FY * FX; p += NT) {
+            const int fy = GR * y0 + p / FX, fx = GR * x0 + p % FX;
+            if (fx < Xf && fy < Yf) store_s(band + lat_at(q, (size_t)B.layer * Yf + fy, fx, Xf), fo[p / FX][p % FX]);
+        }
+        __syncthreads();
+    }
 }
 
-template <class S>
-static void launch_ctfd_prec(hipStream_t s, int GR, CtfdBand B0, CtfdBand B1, int Xc, int Yc, int nzb, double r) {
-    dim3 grd((Xc * Yc + 127) / 128, 2), blk(128);
+template <class S, class P>
+static void launch_iface_prec(hipStream_t s, int GR, IfaceBand B0, IfaceBand B1, int Xc, int Yc, HermiteW<P> H,
+                              P r) {
+    constexpr int TX = 32, TY = 8;
+    dim3 grd(((Xc + TX - 1) / TX) * ((Yc + TY - 1) / TY), 2), blk(TX * TY);
     switch (GR) {
-    case 1: hipLaunchKernelGGL((k_ctfd<S, 1>), grd, blk, 0, s, B0, B1, Xc, Yc, nzb, r); break;
-    case 2: hipLaunchKernelGGL((k_ctfd<S, 2>), grd, blk, 0, s, B0, B1, Xc, Yc, nzb, r); break;
-    case 3: hipLaunchKernelGGL((k_ctfd<S, 3>), grd, blk, 0, s, B0, B1, Xc, Yc, nzb, r); break;
-    default: hipLaunchKernelGGL((k_ctfd<S, 4>), grd, blk, 0, s, B0, B1, Xc, Yc, nzb, r); break;
+    case 1: hipLaunchKernelGGL((k_iface<S, P, 1, TX, TY>), grd, blk, 0, s, B0, B1, Xc, Yc, H, r); break;
+    case 2: hipLaunchKernelGGL((k_iface<S, P, 2, TX, TY>), grd, blk, 0, s, B0, B1, Xc, Yc, H, r); break;
+    case 3: hipLaunchKernelGGL((k_iface<S, P, 3, TX, TY>), grd, blk, 0, s, B0, B1, Xc, Yc, H, r); break;
+    default: hipLaunchKernelGGL((k_iface<S, P, 4, TX, TY>), grd, blk, 0, s, B0, B1, Xc, Yc, H, r); break;
     }
 }
 
 hipError_t launch_interface(hipStream_t s, dnsg_ctx *h, int iter) {
-    const int Xc = h->d.Xc, Yc = h->d.Yc, n = Xc * Yc, GR = h->d.GR;
+    const int Xc = h->d.Xc, Yc = h->d.Yc, GR = h->d.GR;
     // Hermite weights with the reference's expression order (tinterpolate_hrmt.c:20,34)
     const double tmp = (((double)iter) / ((double)GR));
-    HermiteW H;
+    HermiteW<double> H;
     H.h00 = (2. * tmp * tmp * tmp - 3. * tmp * tmp + 1.);
     H.h10 = (tmp * tmp * tmp - 2. * tmp * tmp + tmp);
     H.h01 = (-2. * tmp * tmp * tmp + 3. * tmp * tmp);
     H.h11 = (tmp * tmp * tmp - tmp * tmp);
-    Levels L0 = {h->lev_lo[0], h->lev_lo[1], h->lev_lo[2], h->slots_lo};
-    Levels L1 = {h->lev_up[0], h->lev_up[1], h->lev_up[2], h->slots_up};
-    {
-        dim3 grd((n * NV + 255) / 256, 2), blk(256);
-        hipLaunchKernelGGL(k_hermite, grd, blk, 0, s, L0, L1, n * NV, H);
-        hipLaunchKernelGGL(k_deriv12, grd, blk, 0, s, h->slots_lo, h->slots_up, Xc, Yc);
-        hipLaunchKernelGGL(k_deriv3, grd, blk, 0, s, h->slots_lo, h->slots_up, Xc, Yc);
-    }
     // tau_f/(GR tau_c) written with the reference's variables (omega = 1/tau):
     // tauc/((double)GR*tauf) (ctfdtransfer_opt.c:93)
     const double rescale = h->d.omega_c / ((double)GR * h->d.omega_f);
     const int zds = h->upper.nz;
-    CtfdBand B0 = {h->slots_lo, h->lower.n(), zds - 1};  // lower band: interface is the top stored layer
-    CtfdBand B1 = {h->slots_up, h->upper.n(), 0};        // upper band: interface is stored layer 0
-    if (h->p.precision == DNSG_PREC_FP64) launch_ctfd_prec<double>(s, GR, B0, B1, Xc, Yc, zds, rescale);
-    else if (h->p.precision == DNSG_PREC_FP32) launch_ctfd_prec<float>(s, GR, B0, B1, Xc, Yc, zds, rescale);
-    else launch_ctfd_prec<__half>(s, GR, B0, B1, Xc, Yc, zds, rescale);
+    IfaceBand B0 = {h->lev_lo[0], h->lev_lo[1], h->lev_lo[2], h->lower.n(), zds - 1};  // lower: top stored layer
+    IfaceBand B1 = {h->lev_up[0], h->lev_up[1], h->lev_up[2], h->upper.n(), 0};        // upper: stored layer 0
+    if (h->p.precision == DNSG_PREC_FP64) {
+        launch_iface_prec<double, double>(s, GR, B0, B1, Xc, Yc, H, rescale);
+    } else {
+        HermiteW<float> Hf = {(float)H.h00, (float)H.h10, (float)H.h01, (float)H.h11};
+        if (h->p.precision == DNSG_PREC_FP32) launch_iface_prec<float, float>(s, GR, B0, B1, Xc, Yc, Hf, (float)rescale);
+        else launch_iface_prec<__half, float>(s, GR, B0, B1, Xc, Yc, Hf, (float)rescale);
+    }
     return hipGetLastError();
 }
 
@@ -302,16 +351,14 @@ __global__ __launch_bounds__(256) void k_ftcd(FtcdSide A0, FtcdSide A1, void *co
     if (id >= n) return;
     const int ic = id % Xc, jc = id / Xc;
     const int Xf = GR * Xc, Yf = GR * Yc;
-    const size_t qf = (size_t)Xf * Yf * nzb, qc = (size_t)n * zdc;
     const S *band = static_cast<const S *>(A.band);
     S *cs = static_cast<S *>(coarse);
-    const size_t src = ((size_t)A.layer * Yf + (size_t)GR * jc) * Xf + (size_t)GR * ic;
-    const size_t dst = (size_t)A.kc * n + id;
+    const size_t srow = (size_t)A.layer * Yf + (size_t)GR * jc, drow = (size_t)A.kc * Yc + jc;
     double f[Q];
     unroll<0, Q>([&](auto qq) {
         constexpr int q = decltype(qq)::value;
-        if constexpr (sizeof(S) == 8) f[q] = band[q * qf + src];
-        else f[q] = (double)load_s(band + q * qf + src);
+        if constexpr (sizeof(S) == 8) f[q] = band[lat_at(q, srow, GR * ic, Xf)];
+        else f[q] = (double)load_s(band + lat_at(q, srow, GR * ic, Xf));
     });
     if constexpr (sizeof(S) == 8) {
         const Macro64 m = macro64(f);
@@ -319,7 +366,7 @@ __global__ __launch_bounds__(256) void k_ftcd(FtcdSide A0, FtcdSide A1, void *co
             constexpr int q = decltype(qq)::value;
             const double feq = feq64<q>(m.dens, m.u0, m.u1, m.u2, m.v);
             const double fneq = f[q] - feq;
-            cs[q * qc + dst] = feq + fac * fneq;
+            cs[lat_at(q, drow, ic, Xc)] = feq + fac * fneq;
         });
     } else {
         const double drho = zeroth_moment(f), dens = 1. + drho;
@@ -329,7 +376,7 @@ __global__ __launch_bounds__(256) void k_ftcd(FtcdSide A0, FtcdSide A1, void *co
             constexpr int q = decltype(qq)::value;
             const double eu = edot<q>(u0, u1, u2);
             const double geq = WQ[q] * (drho + dens * (3. * eu + 4.5 * eu * eu - 1.5 * v));
-            store_s(cs + q * qc + dst, (float)(geq + fac * (f[q] - geq)));
+            store_s(cs + lat_at(q, drow, ic, Xc), (float)(geq + fac * (f[q] - geq)));
         });
     }
 }

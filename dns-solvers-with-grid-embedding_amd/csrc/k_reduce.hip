@@ -48,13 +48,13 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double (*lds)[RB / 64]
 }
 
 template <class S>
-__device__ __forceinline__ void node_moments(const S *lat, size_t qs, size_t idx, double &dens, double &ux, double &uy,
-                                             double &uz) {
+__device__ __forceinline__ void node_moments(const S *lat, int nx, size_t row, int x, double &dens, double &ux,
+                                             double &uy, double &uz) {
     double f[Q];
     unroll<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        if constexpr (sizeof(S) == 8) f[q] = lat[q * qs + idx];
-        else f[q] = (double)load_s(lat + q * qs + idx);
+        if constexpr (sizeof(S) == 8) f[q] = lat[lat_at(q, row, x, nx)];
+        else f[q] = (double)load_s(lat + lat_at(q, row, x, nx));
     });
     ux = first_moment<0>(f);
     uy = first_moment<1>(f);
@@ -84,7 +84,6 @@ __global__ __launch_bounds__(RB) void k_row_moments(RedSet R) {
     if (row >= L.ny * L.nz) return;
     __shared__ double lds[NMOM][RB / 64];
     const int z = row / L.ny;
-    const size_t qs = (size_t)L.nx * L.ny * L.nz, row0 = (size_t)row * L.nx;
     const S *lat = static_cast<const S *>(L.lat);
     double acc[NMOM] = {0., 0., 0., 0., 0.};
     double U = 0., V = 0., Wm = 0., D = 0.;
@@ -93,7 +92,7 @@ __global__ __launch_bounds__(RB) void k_row_moments(RedSet R) {
     }
     for (int x = threadIdx.x; x < L.nx; x += RB) {
         double dens, ux, uy, uz;
-        node_moments(lat, qs, row0 + x, dens, ux, uy, uz);
+        node_moments(lat, L.nx, (size_t)row, x, dens, ux, uy, uz);
         if (MODE == 1) {
             acc[0] += dens; acc[1] += ux; acc[2] += ux / dens; acc[3] += uy / dens; acc[4] += uz / dens;
         } else if (MODE == 2) {
@@ -147,6 +146,7 @@ static RedSet make_set(dnsg_ctx *h, const double *means_c, const double *means_l
 // ---- encalc2 ---------------------------------------------------------------------------
 struct EncArgs {
     const double *pc, *pl, *pu;
+    int st;  // doubles per row partial: [0] sum rho, [1] sum rho u_x
     int Yc, zdc, Yf, zds, GR, zdf, NZs, Xf, force_mode;
     double ub, g_const;
     double *force, *diag;
@@ -158,20 +158,20 @@ __global__ __launch_bounds__(RB) void k_fin_encalc2(EncArgs A) {
     const double g3 = 1. * A.GR * A.GR * A.GR, g3h = 0.5 * A.GR * A.GR * A.GR;
     // lower band k = 1 .. zdf-GR-1  <->  stored m = 0 .. zdf-GR-2, weight 1
     const int nl = A.Yf * (A.zdf - A.GR - 1);
-    for (int r = threadIdx.x; r < nl; r += RB) { v[0] += A.pl[r * NMOM + 1]; v[1] += A.pl[r * NMOM + 0]; }
+    for (int r = threadIdx.x; r < nl; r += RB) { v[0] += A.pl[r * A.st + 1]; v[1] += A.pl[r * A.st + 0]; }
     // coarse: every layer, GR^3 (half at k = 0, zdc-1)
     const int nc = A.Yc * A.zdc;
     for (int r = threadIdx.x; r < nc; r += RB) {
         const int z = r / A.Yc;
         const double c = (z == 0 || z == A.zdc - 1) ? g3h : g3;
-        v[0] += c * A.pc[r * NMOM + 1]; v[1] += c * A.pc[r * NMOM + 0];
-        v[2] += c * A.pc[r * NMOM + 1]; v[3] += c * A.pc[r * NMOM + 0];
+        v[0] += c * A.pc[r * A.st + 1]; v[1] += c * A.pc[r * A.st + 0];
+        v[2] += c * A.pc[r * A.st + 1]; v[3] += c * A.pc[r * A.st + 0];
     }
     // upper band k = GR .. zdf-2  <->  stored m = GR .. zds-1, half at k = GR
     const int nu0 = A.Yf * A.GR, nu1 = A.Yf * A.zds;
     for (int r = nu0 + threadIdx.x; r < nu1; r += RB) {
         const double c = (r / A.Yf == A.GR) ? 0.5 : 1.;
-        v[0] += c * A.pu[r * NMOM + 1]; v[1] += c * A.pu[r * NMOM + 0];
+        v[0] += c * A.pu[r * A.st + 1]; v[1] += c * A.pu[r * A.st + 0];
     }
     block_sum<4>(v, lds);
     if (threadIdx.x == 0) {
@@ -190,17 +190,33 @@ __global__ __launch_bounds__(RB) void k_fin_encalc2(EncArgs A) {
     }
 }
 
-hipError_t launch_encalc2(hipStream_t s, dnsg_ctx *h, double *diag_rec) {
-    RedSet R = make_set(h, nullptr, nullptr, nullptr);
-    row_moments<1>(s, h->p.precision, R, 3);
+static EncArgs enc_args(dnsg_ctx *h, const double *pc, const double *pl, const double *pu, int st,
+                        double *diag_rec) {
     EncArgs A;
-    A.pc = R.g[0].part; A.pl = R.g[1].part; A.pu = R.g[2].part;
+    A.pc = pc; A.pl = pl; A.pu = pu; A.st = st;
     A.Yc = h->d.Yc; A.zdc = h->d.zdc; A.Yf = h->d.Yf; A.zds = h->lower.nz; A.GR = h->d.GR; A.zdf = h->d.zdf;
     A.NZs = h->d.NZs; A.Xf = h->d.Xf; A.force_mode = h->p.force_mode; A.ub = h->p.ub; A.g_const = h->p.g_const;
     A.force = h->d_force; A.diag = diag_rec;
+    return A;
+}
+
+// encalc2 from the row sums the collide-stream kernels wrote (RED_W per row)
+hipError_t launch_encalc2_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec) {
+    const double *pc = h->red, *pl = pc + (size_t)h->coarse.ny * h->coarse.nz * RED_W;
+    const double *pu = pl + (size_t)h->lower.ny * h->lower.nz * RED_W;
+    EncArgs A = enc_args(h, pc, pl, pu, RED_W, diag_rec);
     hipLaunchKernelGGL(k_fin_encalc2, dim3(1), dim3(RB), 0, s, A);
     return hipGetLastError();
 }
+
+hipError_t launch_encalc2(hipStream_t s, dnsg_ctx *h, double *diag_rec) {
+    RedSet R = make_set(h, nullptr, nullptr, nullptr);
+    row_moments<1>(s, h->p.precision, R, 3);
+    EncArgs A = enc_args(h, R.g[0].part, R.g[1].part, R.g[2].part, NMOM, diag_rec);
+    hipLaunchKernelGGL(k_fin_encalc2, dim3(1), dim3(RB), 0, s, A);
+    return hipGetLastError();
+}
+
 
 // ---- wall shear stress ---------------------------------------------------------------------
 // per (x, y): lower wall  -2u(k=1) + 3u(k=2) - u(k=3), rho(k=1)
@@ -210,20 +226,19 @@ __global__ __launch_bounds__(RB) void k_wshear_rows(const void *lo, const void *
                                                     double *part) {
     __shared__ double lds[4][RB / 64];
     const int y = blockIdx.x;
-    const size_t qs = (size_t)nx * ny * nz;
     const S *L = static_cast<const S *>(lo), *U = static_cast<const S *>(up);
     double acc[4] = {0., 0., 0., 0.};
     for (int x = threadIdx.x; x < nx; x += RB) {
         double d1, d2, d3, a1, a2, a3, t0, t1;
-        node_moments(L, qs, ((size_t)0 * ny + y) * nx + x, d1, a1, t0, t1);
-        node_moments(L, qs, ((size_t)1 * ny + y) * nx + x, d2, a2, t0, t1);
-        node_moments(L, qs, ((size_t)2 * ny + y) * nx + x, d3, a3, t0, t1);
+        node_moments(L, nx, (size_t)0 * ny + y, x, d1, a1, t0, t1);
+        node_moments(L, nx, (size_t)1 * ny + y, x, d2, a2, t0, t1);
+        node_moments(L, nx, (size_t)2 * ny + y, x, d3, a3, t0, t1);
         const double ux1 = a1 / d1, ux2 = a2 / d2, ux3 = a3 / d3;
         acc[0] += (-2. * ux1 + 3. * ux2 - ux3);
         acc[1] += d1;
-        node_moments(U, qs, ((size_t)(nz - 1) * ny + y) * nx + x, d1, a1, t0, t1);
-        node_moments(U, qs, ((size_t)(nz - 2) * ny + y) * nx + x, d2, a2, t0, t1);
-        node_moments(U, qs, ((size_t)(nz - 3) * ny + y) * nx + x, d3, a3, t0, t1);
+        node_moments(U, nx, (size_t)(nz - 1) * ny + y, x, d1, a1, t0, t1);
+        node_moments(U, nx, (size_t)(nz - 2) * ny + y, x, d2, a2, t0, t1);
+        node_moments(U, nx, (size_t)(nz - 3) * ny + y, x, d3, a3, t0, t1);
         const double vx1 = a1 / d1, vx2 = a2 / d2, vx3 = a3 / d3;
         acc[2] += -(2. * vx1 - 3. * vx2 + vx3);
         acc[3] += d3;
@@ -242,13 +257,39 @@ struct WsArgs {
     double *out;  // [4]: str_up, str_down, stress, pgrad
 };
 
+__device__ void wshear_finish(const WsArgs &A, const double *v);
+
 __global__ __launch_bounds__(RB) void k_fin_wshear(WsArgs A) {
     __shared__ double lds[4][RB / 64];
     double v[4] = {0., 0., 0., 0.};
     for (int r = threadIdx.x; r < A.Yf; r += RB)
         for (int k = 0; k < 4; k++) v[k] += A.part[(size_t)r * 4 + k];
     block_sum<4>(v, lds);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0) wshear_finish(A, v);
+}
+
+// wall stress from the row sums of the first band substep (RED_W per row):
+// sum_x (-2u1 + 3u2 - u3) = -2 S1 + 3 S2 - S3 with S_m = sum_x u_x of layer m
+__global__ __launch_bounds__(RB) void k_fin_wshear_fused(WsArgs A, const double *pl, const double *pu, int nz) {
+    __shared__ double lds[4][RB / 64];
+    double v[4] = {0., 0., 0., 0.};
+    const int Yf = A.Yf;
+    for (int y = threadIdx.x; y < Yf; y += RB) {
+        const double *l0 = pl + ((size_t)0 * Yf + y) * RED_W, *l1 = pl + ((size_t)1 * Yf + y) * RED_W;
+        const double *l2 = pl + ((size_t)2 * Yf + y) * RED_W;
+        const double *u1 = pu + ((size_t)(nz - 1) * Yf + y) * RED_W, *u2 = pu + ((size_t)(nz - 2) * Yf + y) * RED_W;
+        const double *u3 = pu + ((size_t)(nz - 3) * Yf + y) * RED_W;
+        v[0] += (-2. * l0[2] + 3. * l1[2] - l2[2]);
+        v[1] += l0[0];
+        v[2] += -(2. * u1[2] - 3. * u2[2] + u3[2]);
+        v[3] += u3[0];
+    }
+    block_sum<4>(v, lds);
+    if (threadIdx.x == 0) wshear_finish(A, v);
+}
+
+__device__ void wshear_finish(const WsArgs &A, const double *v) {
+    {
         const double mu = (1. / 6.) * (2. / A.omega_f - 1);              // wshearstr.c:181
         const double area = ((double)((A.Xf + 1 - 1.) * (A.Yf + 1 - 1.)));
         double davg = v[1] / area, stdown = v[0] / area;                   // :218-219
@@ -261,6 +302,16 @@ __global__ __launch_bounds__(RB) void k_fin_wshear(WsArgs A) {
         if (A.diag) { A.diag[5] = stup; A.diag[6] = stdown; A.diag[7] = stress; A.diag[8] = pgrad; }
         if (A.out) { A.out[0] = stup; A.out[1] = stdown; A.out[2] = stress; A.out[3] = pgrad; }
     }
+}
+
+hipError_t launch_wshear_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec) {
+    const double *pl = h->red + (size_t)h->coarse.ny * h->coarse.nz * RED_W;
+    const double *pu = pl + (size_t)h->lower.ny * h->lower.nz * RED_W;
+    WsArgs A;
+    A.part = nullptr; A.Xf = h->d.Xf; A.Yf = h->d.Yf; A.NZs = h->d.NZs; A.GR = h->d.GR; A.omega_f = h->d.omega_f;
+    A.force = h->d_force; A.diag = diag_rec; A.out = nullptr;
+    hipLaunchKernelGGL(k_fin_wshear_fused, dim3(1), dim3(RB), 0, s, A, pl, pu, h->lower.nz);
+    return hipGetLastError();
 }
 
 hipError_t launch_wshear(hipStream_t s, dnsg_ctx *h, double *diag_rec) {

@@ -35,6 +35,12 @@ struct LatPair {
     LatArgs l[2];
 };
 
+__device__ __forceinline__ double wave_sum64(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
 template <class S>
 __global__ __launch_bounds__(256) void k_collide_stream(LatPair P, const double *__restrict__ force) {
     const LatArgs &L = P.l[blockIdx.y];
@@ -43,21 +49,33 @@ __global__ __launch_bounds__(256) void k_collide_stream(LatPair P, const double 
     if (row >= ny * nz) return;
     using T = typename ComputeOf<S>::T;
     const int y = row % ny, z = row / ny;
-    const size_t qs = (size_t)nx * ny * nz;
-    const S *__restrict__ cur = static_cast<const S *>(L.cur);
+    const S *__restrict__ cur = static_cast<const S *>(L.cur) + (size_t)row * Q * nx;
     S *__restrict__ nxt = static_cast<S *>(L.nxt);
     const T G = (T)(L.gscale * force[0]);
     const T om = (T)L.omega;
     const bool lo_edge = (z == 0), hi_edge = (z == nz - 1);
     const int ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
-    const size_t row0 = ((size_t)z * ny + y) * nx;
+    const bool red = L.red != nullptr;
+    double r_rho = 0., r_mom = 0., r_u = 0.;
     for (int x = threadIdx.x; x < nx; x += blockDim.x) {
         T f[Q];
-        const size_t base = row0 + x;
         unroll<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            f[q] = load_s(cur + q * qs + base);
+            f[q] = load_s(cur + (size_t)q * nx + x);
         });
+        if (red) {  // moments of the pre-collision state, for the fused reductions
+            double d = 0., m = 0.;
+            if constexpr (sizeof(S) == 8) {
+                m = first_moment<0>(f);
+                d = zeroth_moment(f);
+            } else {
+                m = (double)first_moment<0>(f);
+                d = 1. + (double)zeroth_moment(f);
+            }
+            r_rho += d;
+            r_mom += m;
+            r_u += m / d;
+        }
         collide_any(f, G, om);
         const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
         unroll<0, Q>([&](auto qc) {
@@ -73,16 +91,34 @@ __global__ __launch_bounds__(256) void k_collide_stream(LatPair P, const double 
                         const int sld = wc[(size_t)yt * nx + xt];
                         const int sld1 = wc[(size_t)y * nx + x];
                         if (sld == -1 && sld1 == -1)
-                            store_s(nxt + OPP[q] * qs + base, f[q]);
+                            store_s(nxt + lat_at(OPP[q], (size_t)row, x, nx), f[q]);
                         else
-                            store_s(nxt + MIR[q] * qs + row0 - (size_t)y * nx + (size_t)yt * nx + xt, f[q]);
+                            store_s(nxt + lat_at(MIR[q], (size_t)z * ny + yt, xt, nx), f[q]);
                     }
                     return;
                 }
             }
-            const size_t dst = ((size_t)(z + EZ[q]) * ny + yt) * nx + xt;
-            store_s(nxt + q * qs + dst, f[q]);
+            const int zt = z + EZ[q];
+            if ((zt == 0 && L.skip_lo) || (zt == nz - 1 && L.skip_hi)) return;
+            store_s(nxt + lat_at(q, (size_t)zt * ny + yt, xt, nx), f[q]);
         });
+    }
+    if (red) {  // uniform per workgroup
+        __shared__ double lds[3][4];
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        r_rho = wave_sum64(r_rho);
+        r_mom = wave_sum64(r_mom);
+        r_u = wave_sum64(r_u);
+        if (lane == 0) { lds[0][wid] = r_rho; lds[1][wid] = r_mom; lds[2][wid] = r_u; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int nw = (blockDim.x + 63) >> 6;
+            double a = 0., b = 0., c = 0.;
+            for (int w = 0; w < nw; w++) { a += lds[0][w]; b += lds[1][w]; c += lds[2][w]; }
+            L.red[(size_t)row * RED_W + 0] = a;
+            L.red[(size_t)row * RED_W + 1] = b;
+            L.red[(size_t)row * RED_W + 2] = c;
+        }
     }
 }
 
@@ -158,11 +194,11 @@ __global__ __launch_bounds__(256) void k_init(InitArgs A) {
             f[q] = feq64<q>(1., u0, u1, u2, v);
         });
     }
-    const size_t qs = n;
+    const size_t row = (size_t)z * A.ny + y;
     unroll<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        if constexpr (sizeof(S) == 8) store_s((double *)dst + q * qs + id, f[q]);
-        else store_s(dst + q * qs + id, (float)(f[q] - WQ[q]));
+        if constexpr (sizeof(S) == 8) store_s((double *)dst + lat_at(q, row, x, A.nx), f[q]);
+        else store_s(dst + lat_at(q, row, x, A.nx), (float)(f[q] - WQ[q]));
     });
 }
 
@@ -189,12 +225,12 @@ hipError_t launch_init_lattice(hipStream_t s, dnsg_ctx *h, int grid, void *dst) 
 template <class S>
 __global__ __launch_bounds__(256) void k_aos_to_soa(const double *__restrict__ aos, S *__restrict__ soa, int nx,
                                                     int ny, int nz_aos, int z0, int nz) {
-    const size_t qs = (size_t)nx * ny * nz;
+    const size_t n = (size_t)nx * ny * nz * Q;
     const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= qs * Q) return;
-    const int q = (int)(id / qs);
-    const size_t r = id % qs;
-    const int x = (int)(r % nx), y = (int)((r / nx) % ny), z = (int)(r / ((size_t)nx * ny));
+    if (id >= n) return;
+    const int x = (int)(id % nx), q = (int)((id / nx) % Q);
+    const size_t row = id / ((size_t)nx * Q);
+    const int y = (int)(row % ny), z = (int)(row / ny);
     const double v = aos[(((size_t)x * ny + y) * nz_aos + (z + z0)) * Q + q];
     if constexpr (sizeof(S) == 8) soa[id] = v;
     else store_s(soa + id, (float)(v - WQ[q]));
@@ -203,12 +239,12 @@ __global__ __launch_bounds__(256) void k_aos_to_soa(const double *__restrict__ a
 template <class S>
 __global__ __launch_bounds__(256) void k_soa_to_aos(const S *__restrict__ soa, double *__restrict__ aos, int nx,
                                                     int ny, int nz_aos, int z0, int nz) {
-    const size_t qs = (size_t)nx * ny * nz;
+    const size_t n = (size_t)nx * ny * nz * Q;
     const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= qs * Q) return;
-    const int q = (int)(id / qs);
-    const size_t r = id % qs;
-    const int x = (int)(r % nx), y = (int)((r / nx) % ny), z = (int)(r / ((size_t)nx * ny));
+    if (id >= n) return;
+    const int x = (int)(id % nx), q = (int)((id / nx) % Q);
+    const size_t row = id / ((size_t)nx * Q);
+    const int y = (int)(row % ny), z = (int)(row / ny);
     double v;
     if constexpr (sizeof(S) == 8) v = soa[id];
     else v = (double)load_s(soa + id) + WQ[q];

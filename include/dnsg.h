@@ -198,6 +198,11 @@ int dnsg_run_timed(dnsg_t *h, int n, dnsg_diag *diag, double *elapsed_ms);
  * (+ fine->coarse injection after the last substep). */
 int dnsg_kernel_times(dnsg_t *h, int n, double *ms);
 
+/* Instrumentation (no counterpart in the reference): achievable HBM bandwidth of
+ * a plain kernel over a buffer of `bytes` -- kind 0 copy (read + write),
+ * 1 read-only, 2 write-only -- best of `reps` launches, in GB/s. */
+int dnsg_hbm_probe(size_t bytes, int kind, int reps, double *gbs);
+
 /* ---- reference-granular entry points (one per fun_defs.h function) ---------- */
 
 /* encalc2.c:17-147 -> V.force[0] (flux mode) and the flowrate/density scalars. */
