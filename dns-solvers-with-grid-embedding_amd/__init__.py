@@ -67,6 +67,21 @@ class Diag(ctypes.Structure):
 # exported entry points and their signatures (all return int status)
 _P = ctypes.c_void_p
 _DP = ctypes.POINTER(ctypes.c_double)
+class QueueOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_active", ctypes.c_int), ("check_every", ctypes.c_int),
+                ("window", ctypes.c_int), ("tol", ctypes.c_double), ("max_steps", ctypes.c_long)]
+
+
+class SlipResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int), ("precision", ctypes.c_int),
+                ("steps", ctypes.c_long), ("ubulk", ctypes.c_double), ("u_noslip", ctypes.c_double),
+                ("slip_length", ctypes.c_double), ("t_arrival", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double)]
+
+    def asdict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 SIGNATURES = {
     "dnsg_params_default": [ctypes.POINTER(Params)],
     "dnsg_dims_from_params": [ctypes.POINTER(Params), ctypes.POINTER(Dims)],
@@ -94,6 +109,13 @@ SIGNATURES = {
     "dnsg_stats_average": [_P, _DP, ctypes.POINTER(ctypes.c_long)],
     "dnsg_stats_reset": [_P],
     "dnsg_velocity_snapshot": [_P, ctypes.c_int, _DP, _DP],
+    "dnsg_queue_opts_default": [ctypes.POINTER(QueueOpts)],
+    "dnsg_queue_create": [ctypes.POINTER(QueueOpts), ctypes.POINTER(_P)],
+    "dnsg_queue_destroy": [_P],
+    "dnsg_queue_submit": [_P, ctypes.POINTER(Params), ctypes.c_uint64, ctypes.c_double],
+    "dnsg_queue_advance": [_P, ctypes.c_int, ctypes.POINTER(SlipResult), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+    "dnsg_queue_info": [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                        ctypes.POINTER(ctypes.c_size_t)],
 }
 
 _lib = None
@@ -114,6 +136,12 @@ def load(path=LIB_PATH):
         fn.restype = ctypes.c_int
     lib.dnsg_last_error.argtypes = [_P]
     lib.dnsg_last_error.restype = ctypes.c_char_p
+    lib.dnsg_queue_last_error.argtypes = [_P]
+    lib.dnsg_queue_last_error.restype = ctypes.c_char_p
+    lib.dnsg_queue_clock.argtypes = [_P]
+    lib.dnsg_queue_clock.restype = ctypes.c_double
+    lib.dnsg_solve_bytes.argtypes = [ctypes.POINTER(Params)]
+    lib.dnsg_solve_bytes.restype = ctypes.c_size_t
     _lib = lib
     return lib
 
@@ -283,6 +311,62 @@ class Solver:
         self._check(self.lib.dnsg_fexchange(self.h))
 
 
+class Queue:
+    """Serving queue of steady slip-length solves (dnsg_queue_*): submit channels
+    as they arrive, advance, collect finished solves."""
+
+    def __init__(self, **opts):
+        self.lib = load()
+        self.q = None
+        o = QueueOpts()
+        self.lib.dnsg_queue_opts_default(ctypes.byref(o))
+        for k, v in opts.items():
+            if not hasattr(o, k):
+                raise KeyError(f"unknown queue option {k}")
+            setattr(o, k, type(getattr(o, k))(v))
+        self.opts = o
+        q = _P()
+        rc = self.lib.dnsg_queue_create(ctypes.byref(o), ctypes.byref(q))
+        self.q = q
+        self._check(rc)
+
+    def _check(self, rc):
+        if rc:
+            raise DnsgError(rc, self.lib.dnsg_queue_last_error(self.q).decode())
+
+    def close(self):
+        if self.q:
+            self.lib.dnsg_queue_destroy(self.q)
+            self.q = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clock(self):
+        return self.lib.dnsg_queue_clock(self.q)
+
+    def submit(self, params, rid, t_arrival=-1.0):
+        self._check(self.lib.dnsg_queue_submit(self.q, ctypes.byref(params), rid, t_arrival))
+
+    def advance(self, steps, cap=256):
+        buf = (SlipResult * cap)()
+        nd = ctypes.c_int()
+        self._check(self.lib.dnsg_queue_advance(self.q, steps, buf, cap, ctypes.byref(nd)))
+        return [buf[i].asdict() for i in range(nd.value)]
+
+    def info(self):
+        p, a, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+        self._check(self.lib.dnsg_queue_info(self.q, ctypes.byref(p), ctypes.byref(a), ctypes.byref(b)))
+        return dict(pending=p.value, active=a.value, resident_bytes=b.value)
+
+
+def solve_bytes(params):
+    return load().dnsg_solve_bytes(ctypes.byref(params))
+
+
 def hbm_probe(nbytes=2 << 30, kind=0, reps=5):
     """achievable HBM bandwidth (GB/s): kind 0 copy, 1 read, 2 write"""
     g = ctypes.c_double()
@@ -294,3 +378,60 @@ def hbm_probe(nbytes=2 << 30, kind=0, reps=5):
 
 def diag_array(d, n):
     return {f: np.array([getattr(d[i], f) for i in range(n)]) for f in DIAG_FIELDS}
+
+
+def slip_request(xdv, ydv, zdv, gratio, pattern, period, gas_fraction, re_b, u0=0.05, precision="fp32", seed=0):
+    """Parameters of one steady slip-length solve.
+
+    Channel of coarse size xdv (streamwise) x ydv (spanwise) x zdv (wall-normal,
+    including the two overlap layers); gratio 2 refines four coarse cells at each
+    wall (zds = 8), gratio 1 keeps the near-wall band at the coarse resolution
+    (zds = 4).  The wall pattern (stripes "rslip" or square posts "pslip") has
+    period `period` fine cells and the given gas fraction.  The body force G gives
+    the no-slip bulk velocity u0, and nu follows from Re_b = u0 H / nu with H the
+    wall-to-wall height (the reference's convention, main:447).  The start is the
+    no-slip laminar profile."""
+    GR = int(gratio)
+    zds = 8 if GR == 2 else max(4, GR + 3)
+    zdf = zds + 1
+    NZs = 2 * zdf + GR * (zdv - 4) + GR - 1
+    H = (NZs - 2.0) / GR
+    nu = u0 * H / re_b
+    tau_c = 3.0 * nu + 0.5
+    G = 12.0 * nu * u0 / (H * H)
+    kw = dict(xdv=xdv, ydv=ydv, zdv=zdv, gratio=GR, zds=zds, tau_c=tau_c, ub=u0, force_mode="const", g_const=G,
+              precision=precision, init_kind="laminar_noise", seed=seed, u_max=1.5 * u0, noise=0.0, CFL=1.0)
+    P = int(period)
+    if pattern == "rslip":
+        g = min(P - 1, max(1, int(round(gas_fraction * P))))
+        kw.update(pattern="rslip", gyplussize=g, wyplussize=P - g)
+    elif pattern == "pslip":
+        w = min(P - 1, max(1, int(round(P * (1.0 - gas_fraction) ** 0.5))))
+        kw.update(pattern="pslip", gyplussize=P - w, wyplussize=w, gxplussize=P - w, wxplussize=w)
+    else:
+        kw.update(pattern="none")
+    return make_params(**kw)
+
+
+def slip_workload(n, seed, x_range=(32, 256), y_range=(32, 256), z_range=(16, 64), re_range=(10.0, 200.0),
+                  precision="fp32"):
+    """n slip-length requests drawn from one seed: coarse streamwise and spanwise
+    sizes (multiples of 16) and wall-normal size uniform in their ranges, bands
+    (gratio 2) on a random half, stripes or posts, gas fraction in [0.2, 0.9],
+    period 8..64 fine cells dividing the patterned extents, Re_b uniform."""
+    import random
+    rng = random.Random(seed)
+    out = []
+    for i in range(n):
+        xdv = rng.randrange(x_range[0], x_range[1] + 1, 16)
+        ydv = rng.randrange(y_range[0], y_range[1] + 1, 16)
+        zdv = rng.randint(*z_range)
+        GR = 2 if rng.random() < 0.5 else 1
+        pattern = "rslip" if rng.random() < 0.5 else "pslip"
+        periods = [P for P in (8, 16, 32, 64) if (GR * ydv) % P == 0 and (pattern == "rslip" or (GR * xdv) % P == 0)]
+        P = rng.choice(periods)
+        phi = rng.uniform(0.2, 0.9)
+        re_b = rng.uniform(*re_range)
+        desc = dict(id=i, xdv=xdv, ydv=ydv, zdv=zdv, gratio=GR, pattern=pattern, period=P, gas_fraction=phi, re_b=re_b)
+        out.append((desc, slip_request(xdv, ydv, zdv, GR, pattern, P, phi, re_b, precision=precision, seed=seed + i)))
+    return out

@@ -134,6 +134,10 @@ int ensure_pinned(dnsg_t *h, size_t n) {
     return DNSG_OK;
 }
 
+}  // namespace
+
+namespace dnsg {
+
 LatArgs coarse_args(dnsg_t *h) {
     LatArgs L;
     L.cur = h->coarse.c();
@@ -146,6 +150,7 @@ LatArgs coarse_args(dnsg_t *h) {
     L.skip_lo = 1;  // coarse k = 0, zdc-1 are rewritten by the fine->coarse injection
     L.skip_hi = 1;
     L.red = nullptr;
+    L.force = h->d_force;
     return L;
 }
 
@@ -167,12 +172,13 @@ void band_args(dnsg_t *h, LatArgs *L) {
         L[b].skip_lo = b == 0 ? 0 : 1;
         L[b].skip_hi = b == 0 ? 1 : 0;
         L[b].red = nullptr;
+        L[b].force = h->d_force;
     }
 }
 
 double *red_coarse(dnsg_t *h) { return h->red; }
-double *red_lower(dnsg_t *h) { return h->red + (size_t)h->coarse.ny * h->coarse.nz * RED_W; }
-double *red_upper(dnsg_t *h) { return red_lower(h) + (size_t)h->lower.ny * h->lower.nz * RED_W; }
+double *red_lower(dnsg_t *h) { return h->red + (size_t)h->coarse.ny * h->coarse.nz * cs_chunks(h->coarse.nx) * RED_W; }
+double *red_upper(dnsg_t *h) { return red_lower(h) + (size_t)h->lower.ny * h->lower.nz * cs_chunks(h->lower.nx) * RED_W; }
 
 void rotate_levels(dnsg_t *h) {  // xcomputations.c:548-556
     void *t = h->lev_lo[2];
@@ -181,9 +187,13 @@ void rotate_levels(dnsg_t *h) {  // xcomputations.c:548-556
     h->lev_up[2] = h->lev_up[1]; h->lev_up[1] = h->lev_up[0]; h->lev_up[0] = t;
 }
 
+}  // namespace dnsg
+
+namespace {
+
 int do_xcomputations(dnsg_t *h) {
     LatArgs L = coarse_args(h);
-    HIPCHK(h, launch_collide_stream(h->stream, &L, 1, h->d_force, h->p.precision));
+    HIPCHK(h, launch_collide_stream(h->stream, &L, 1, h->p.precision));
     rotate_levels(h);
     HIPCHK(h, launch_sample(h->stream, h, h->coarse.n(), h->lev_lo[0], h->lev_up[0]));
     return DNSG_OK;
@@ -192,7 +202,7 @@ int do_xcomputations(dnsg_t *h) {
 int do_xfcomputations(dnsg_t *h) {
     LatArgs L[2];
     band_args(h, L);
-    HIPCHK(h, launch_collide_stream(h->stream, L, 2, h->d_force, h->p.precision));
+    HIPCHK(h, launch_collide_stream(h->stream, L, 2, h->p.precision));
     return DNSG_OK;
 }
 
@@ -227,7 +237,7 @@ int do_step(dnsg_t *h, double *rec, SecTimer *T = nullptr) {
     {
         LatArgs L = coarse_args(h);
         if (!flux) L.red = red_coarse(h);
-        HIPCHK(h, launch_collide_stream(h->stream, &L, 1, h->d_force, h->p.precision));
+        HIPCHK(h, launch_collide_stream(h->stream, &L, 1, h->p.precision));
         if (T) T->mark(h->stream, SEC_COARSE);
         rotate_levels(h);
         HIPCHK(h, launch_sample(h->stream, h, h->coarse.n(), h->lev_lo[0], h->lev_up[0]));
@@ -240,7 +250,7 @@ int do_step(dnsg_t *h, double *rec, SecTimer *T = nullptr) {
             L[0].red = red_lower(h);
             L[1].red = red_upper(h);
         }
-        HIPCHK(h, launch_collide_stream(h->stream, L, 2, h->d_force, h->p.precision));
+        HIPCHK(h, launch_collide_stream(h->stream, L, 2, h->p.precision));
         if (T) T->mark(h->stream, SEC_BAND);
         if (iter == 1) {
             if (!flux) HIPCHK(h, launch_encalc2_fused(h->stream, h, rec));
@@ -425,18 +435,23 @@ int dnsg_destroy(dnsg_t *h) {
         if (h->lev_lo[l]) (void)hipFree(h->lev_lo[l]);
         if (h->lev_up[l]) (void)hipFree(h->lev_up[l]);
     }
-    void *ptrs[] = {h->wall_lo, h->wall_up, h->part, h->red, h->d_force, h->d_diag, h->d_red, h->d_acc};
+    void *ptrs[] = {h->wall_lo, h->wall_up, h->part, h->red, h->seg, h->d_force, h->d_diag, h->d_red, h->d_acc};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (h->h_pinned) (void)hipHostFree(h->h_pinned);
     free(h->host_wall_lo);
     free(h->host_wall_up);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return DNSG_OK;
 }
 
-int dnsg_create(const dnsg_params *p, dnsg_t **out) {
+int dnsg_create(const dnsg_params *p, dnsg_t **out) { return dnsg::create_ctx(p, nullptr, out); }
+
+}  // extern "C"
+
+// a handle on the caller's stream (ext) or on a stream of its own (ext = null)
+int dnsg::create_ctx(const dnsg_params *p, hipStream_t ext, dnsg_t **out) {
     if (!p || !out) return DNSG_ERR_ARG;
     *out = nullptr;
     dnsg_t *h = new (std::nothrow) dnsg_ctx();
@@ -449,7 +464,12 @@ int dnsg_create(const dnsg_params *p, dnsg_t **out) {
     }
     *out = h;
     HIPCHK(h, hipGetDevice(&h->device));
-    HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    if (ext) {
+        h->stream = ext;
+        h->own_stream = false;
+    } else {
+        HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    }
     h->elem = p->precision == DNSG_PREC_FP64 ? 8 : (p->precision == DNSG_PREC_FP32 ? 4 : 2);
     const dnsg_dims &d = h->d;
     h->coarse.nx = d.Xc; h->coarse.ny = d.Yc; h->coarse.nz = d.zdc;
@@ -477,9 +497,11 @@ int dnsg_create(const dnsg_params *p, dnsg_t **out) {
     h->part_cap = partial_doubles(h);
     HIPCHK(h, hipMalloc(&h->part, h->part_cap * 8));
     {
-        const size_t nred = ((size_t)d.Yc * d.zdc + 2 * (size_t)d.Yf * (d.zdf - 1)) * RED_W;
+        const size_t nred =
+            ((size_t)d.Yc * d.zdc * cs_chunks(d.Xc) + 2 * (size_t)d.Yf * (d.zdf - 1) * cs_chunks(d.Xf)) * RED_W;
         HIPCHK(h, hipMalloc(&h->red, nred * 8));
         HIPCHK(h, hipMemsetAsync(h->red, 0, nred * 8, h->stream));
+        HIPCHK(h, hipMalloc(&h->seg, enc_seg_doubles(h) * 8));
     }
     HIPCHK(h, hipMalloc(&h->d_force, 4 * 8));
     HIPCHK(h, hipMemsetAsync(h->d_force, 0, 4 * 8, h->stream));
@@ -499,6 +521,8 @@ int dnsg_create(const dnsg_params *p, dnsg_t **out) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return DNSG_OK;
 }
+
+extern "C" {
 
 static int set_force(dnsg_t *h, double g) {
     h->force = g;

@@ -43,11 +43,32 @@ struct LatArgs {
     double omega;          // 1/tau
     int skip_lo, skip_hi;  // do not write into layer 0 / nz-1 (rewritten by the interface)
     double *red;           // if set: per-row sums of the pre-collision state, RED_W per row
+    const double *force;   // V.force[0] of the solve this lattice belongs to
 };
 
-// per-row sums written by the collide-stream kernels when LatArgs::red is set:
-// sum rho, sum rho u_x, sum u_x (u_x = rho u_x / rho)
+// Several independent work items in one launch: item i owns the workgroups
+// [first[i], first[i+1]) of a one-dimensional grid.  The table travels in the
+// kernel arguments, so a launch needs no device-side setup.
+constexpr int BATCH_MAX = 16;
+template <class A>
+struct Batch {
+    static_assert(sizeof(A) * BATCH_MAX + 4 * (BATCH_MAX + 2) <= 4096, "batch table exceeds the kernel-argument limit");
+    A it[BATCH_MAX];
+    int first[BATCH_MAX + 1];
+    int n;
+};
+__device__ __forceinline__ int batch_find(const int *first, int n, int b) {
+    int i = 0;
+    while (i + 1 < n && b >= first[i + 1]) i++;
+    return i;
+}
+
+// per-(row, chunk) sums written by the collide-stream kernels when LatArgs::red
+// is set: sum rho, sum rho u_x, sum u_x (u_x = rho u_x / rho)
 constexpr int RED_W = 3;
+// threads (= nodes) per collide-stream workgroup and chunks per lattice row
+constexpr int CS_THREADS = 256;
+__host__ __device__ __forceinline__ int cs_chunks(int nx) { return (nx + CS_THREADS - 1) / CS_THREADS; }
 
 struct Grid {
     void *buf[2] = {nullptr, nullptr};  // A/B lattices
@@ -67,12 +88,14 @@ constexpr int MAX_PARTIAL_Q = 5;
 
 }  // namespace dnsg
 
+typedef struct dnsg_ctx dnsg_t;
 struct dnsg_ctx {
     dnsg_params p;
     dnsg_dims d;
     std::string err;
     int device = 0;
     hipStream_t stream = nullptr;
+    bool own_stream = true;  // false: the stream belongs to a queue serving many handles
     size_t elem = 8;  // bytes per stored distribution value
 
     dnsg::Grid coarse, lower, upper;
@@ -84,7 +107,8 @@ struct dnsg_ctx {
 
     // reductions
     double *part = nullptr;     // row partials of the explicit reduction passes
-    double *red = nullptr;      // row sums written by the collide-stream kernels (RED_W per row)
+    double *red = nullptr;      // sums written by the collide-stream kernels (RED_W per row chunk)
+    double *seg = nullptr;      // first-stage segment sums of the flux reduction
     size_t part_cap = 0;        // doubles
     double *d_force = nullptr;  // device copy of V.force[0]
     double *d_diag = nullptr;   // DIAG_WORDS per step
@@ -109,8 +133,18 @@ struct dnsg_ctx {
 
 namespace dnsg {
 
+// handle set-up and the per-step argument builders shared by the single-solve
+// step and the serving queue
+int create_ctx(const dnsg_params *p, hipStream_t ext, dnsg_t **out);
+LatArgs coarse_args(dnsg_t *h);
+void band_args(dnsg_t *h, LatArgs *L);
+double *red_coarse(dnsg_t *h);
+double *red_lower(dnsg_t *h);
+double *red_upper(dnsg_t *h);
+void rotate_levels(dnsg_t *h);
+
 // launchers (stream-ordered, no host synchronisation)
-hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, const double *force, int prec);
+hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, int prec);
 hipError_t launch_init_lattice(hipStream_t s, dnsg_ctx *h, int grid, void *dst);
 hipError_t launch_aos_to_soa(hipStream_t s, const double *aos, void *soa, int nx, int ny, int nz_aos, int z0,
                              int nz, int prec);
@@ -119,6 +153,14 @@ hipError_t launch_soa_to_aos(hipStream_t s, const void *soa, double *aos, int nx
 hipError_t launch_sample(hipStream_t s, dnsg_ctx *h, const void *coarse_state, void *lo, void *up);
 hipError_t launch_interface(hipStream_t s, dnsg_ctx *h, int iter);
 hipError_t launch_ftcd(hipStream_t s, dnsg_ctx *h);
+// the same kernels over several solves of one precision (one launch per
+// BATCH_MAX items); the interface batch needs one gratio
+hipError_t launch_sample_many(hipStream_t s, dnsg_ctx *const *hs, int n);
+hipError_t launch_interface_many(hipStream_t s, dnsg_ctx *const *hs, int n, int iter);
+hipError_t launch_ftcd_many(hipStream_t s, dnsg_ctx *const *hs, int n);
+// bulk momentum and density of each solve from its fused collide-stream sums:
+// out[2 slot[i]] = <rho u_x>, out[2 slot[i] + 1] = <rho> (encalc2.c:109-131 weights)
+hipError_t launch_flux_many(hipStream_t s, dnsg_ctx *const *hs, int n, double *out, const int *slot);
 hipError_t launch_encalc2(hipStream_t s, dnsg_ctx *h, double *diag_rec);
 hipError_t launch_encalc2_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec);
 hipError_t launch_wshear_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec);
@@ -127,5 +169,6 @@ hipError_t launch_mass(hipStream_t s, dnsg_ctx *h, double *out);
 hipError_t launch_statistics(hipStream_t s, dnsg_ctx *h, double *out9nz);
 hipError_t launch_fold_stats(hipStream_t s, double *acc, const double *st, int NZ);
 size_t partial_doubles(const dnsg_ctx *h);
+size_t enc_seg_doubles(const dnsg_ctx *h);
 
 }  // namespace dnsg

@@ -16,6 +16,9 @@
 // so the results are bitwise those of the reference.
 // Shifted tiers (lattices hold g = f - w): FP32 planes and arithmetic, slot 0
 // holds rho - 1 instead of rho, and f_neq = g - (feq - w).
+#include <algorithm>
+#include <vector>
+
 #include "internal.hpp"
 #include "lattice.hpp"
 
@@ -24,15 +27,27 @@ namespace dnsg {
 constexpr int NV = 23;  // rho, u_x, u_y, u_z, f_neq[19]
 
 // ---- sampling --------------------------------------------------------------------
+struct SampleItem {
+    const void *cs;   // coarse post-stream lattice
+    void *plo, *pup;  // newest plane level of each side
+    int nx, ny, k_up; // coarse extents; layers 1 and k_up = zdc-2 are sampled
+};
+
+// item i: workgroups [0, nb) sample the lower side, [nb, 2 nb) the upper side
 template <class S, class P>
-__global__ __launch_bounds__(256) void k_sample(const S *__restrict__ cs, int nx, int ny, int k_lo, int k_up,
-                                                P *__restrict__ plo, P *__restrict__ pup) {
-    const int n = nx * ny;
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_sample(Batch<SampleItem> B) {
+    const int it = batch_find(B.first, B.n, blockIdx.x);
+    const SampleItem &A = B.it[it];
+    const int n = A.nx * A.ny, nb = (n + 255) / 256;
+    const int lb = blockIdx.x - B.first[it];
+    const int side = lb >= nb ? 1 : 0;
+    const int id = (lb - side * nb) * 256 + threadIdx.x;
     if (id >= n) return;
-    const int k = blockIdx.y == 0 ? k_lo : k_up;
-    P *pl = blockIdx.y == 0 ? plo : pup;
-    const size_t row = (size_t)k * ny + id / nx;
+    const int k = side == 0 ? 1 : A.k_up;
+    P *pl = static_cast<P *>(side == 0 ? A.plo : A.pup);
+    const S *cs = static_cast<const S *>(A.cs);
+    const int nx = A.nx;
+    const size_t row = (size_t)k * A.ny + id / nx;
     const int x = id % nx;
     P f[Q];
     unroll<0, Q>([&](auto qc) {
@@ -62,21 +77,50 @@ __global__ __launch_bounds__(256) void k_sample(const S *__restrict__ cs, int nx
     }
 }
 
+// fill a batch table from items [i0, i0 + n) and return its workgroup count
+template <class A, class F>
+static int fill_batch(Batch<A> &B, const A *items, int n, F blocks_of) {
+    B.n = n;
+    int blocks = 0;
+    for (int i = 0; i < BATCH_MAX; i++) {
+        B.it[i] = items[i < n ? i : 0];
+        B.first[i] = blocks;
+        if (i < n) blocks += blocks_of(B.it[i]);
+    }
+    B.first[BATCH_MAX] = blocks;
+    return blocks;
+}
+
+static hipError_t sample_items(hipStream_t s, int prec, const SampleItem *items, int n) {
+    for (int i0 = 0; i0 < n; i0 += BATCH_MAX) {
+        Batch<SampleItem> B;
+        const int blocks = fill_batch(B, items + i0, std::min(BATCH_MAX, n - i0),
+                                      [](const SampleItem &a) { return 2 * ((a.nx * a.ny + 255) / 256); });
+        if (prec == DNSG_PREC_FP64)
+            hipLaunchKernelGGL((k_sample<double, double>), dim3(blocks), dim3(256), 0, s, B);
+        else if (prec == DNSG_PREC_FP32)
+            hipLaunchKernelGGL((k_sample<float, float>), dim3(blocks), dim3(256), 0, s, B);
+        else
+            hipLaunchKernelGGL((k_sample<__half, float>), dim3(blocks), dim3(256), 0, s, B);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 hipError_t launch_sample(hipStream_t s, dnsg_ctx *h, const void *coarse_state, void *lo, void *up) {
     const Grid &g = h->coarse;
-    const int n = g.nx * g.ny;
-    dim3 grd((n + 255) / 256, 2), blk(256);
-    const int prec = h->p.precision;
-    if (prec == DNSG_PREC_FP64)
-        hipLaunchKernelGGL((k_sample<double, double>), grd, blk, 0, s, (const double *)coarse_state, g.nx, g.ny, 1,
-                           g.nz - 2, (double *)lo, (double *)up);
-    else if (prec == DNSG_PREC_FP32)
-        hipLaunchKernelGGL((k_sample<float, float>), grd, blk, 0, s, (const float *)coarse_state, g.nx, g.ny, 1,
-                           g.nz - 2, (float *)lo, (float *)up);
-    else
-        hipLaunchKernelGGL((k_sample<__half, float>), grd, blk, 0, s, (const __half *)coarse_state, g.nx, g.ny, 1,
-                           g.nz - 2, (float *)lo, (float *)up);
-    return hipGetLastError();
+    const SampleItem a = {coarse_state, lo, up, g.nx, g.ny, g.nz - 2};
+    return sample_items(s, h->p.precision, &a, 1);
+}
+
+hipError_t launch_sample_many(hipStream_t s, dnsg_ctx *const *hs, int n) {
+    std::vector<SampleItem> v(n);
+    for (int i = 0; i < n; i++) {
+        const Grid &g = hs[i]->coarse;
+        v[i] = {g.n(), hs[i]->lev_lo[0], hs[i]->lev_up[0], g.nx, g.ny, g.nz - 2};
+    }
+    return n ? sample_items(s, hs[0]->p.precision, v.data(), n) : hipSuccess;
 }
 
 // ---- helpers ---------------------------------------------------------------------------
@@ -167,15 +211,16 @@ __device__ __forceinline__ P edot_rt(int q, P u0, P u1, P u2) {
 //   4. keeps rho and u in registers; for a distribution value writes
 //      f = feq + tau_f/(GR tau_c) f_neq through an LDS tile so that the
 //      interface-layer rows are stored contiguously.
-struct IfaceBand {
+struct IfaceItem {
     const void *ln, *lp, *lpp;  // newest, previous, oldest plane levels [23][Yc][Xc]
     void *band;                 // band post-stream lattice
     int layer;                  // stored interface layer
+    int Xc, Yc;
+    double rescale;             // tau_f / (GR tau_c)
 };
 
 template <class S, class P, int GR, int TX, int TY>
-__global__ __launch_bounds__(TX *TY) void k_iface(IfaceBand B0, IfaceBand B1, int Xc, int Yc, HermiteW<P> H,
-                                                  P rescale) {
+__global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<P> H) {
     constexpr int NT = TX * TY, HX = TX + 5, HY = TY + 5, NP = GR * GR, FX = GR * TX, FY = GR * TY;
     constexpr int NH = (HX * HY + NT - 1) / NT;  // halo points per thread
     __shared__ P s0[HY][HX];          // value, halo 2 on each side (+1 corner)
@@ -183,12 +228,16 @@ __global__ __launch_bounds__(TX *TY) void k_iface(IfaceBand B0, IfaceBand B1, in
     __shared__ P s2[TY + 1][HX];      // d/dy at the corners and the halo columns
     __shared__ P s3[TY + 1][TX + 1];  // d2/dxdy at the corners
     __shared__ P fo[FY][FX];          // output tile of one direction
-    const IfaceBand &B = blockIdx.y == 0 ? B0 : B1;
+    const int itm = batch_find(BT.first, BT.n, blockIdx.x);
+    const IfaceItem &B = BT.it[itm];
+    const int lb = blockIdx.x - BT.first[itm];
+    const int Xc = B.Xc, Yc = B.Yc;
+    const P rescale = (P)B.rescale;
     const P *__restrict__ Ln = static_cast<const P *>(B.ln);
     const P *__restrict__ Lp = static_cast<const P *>(B.lp);
     const P *__restrict__ Lpp = static_cast<const P *>(B.lpp);
     const int ntx = (Xc + TX - 1) / TX;
-    const int x0 = (blockIdx.x % ntx) * TX, y0 = (blockIdx.x / ntx) * TY;
+    const int x0 = (lb % ntx) * TX, y0 = (lb / ntx) * TY;
     const int tid = threadIdx.x, cx = tid % TX, cy = tid / TX;
     const int n = Xc * Yc, Xf = GR * Xc, Yf = GR * Yc;
     S *band = static_cast<S *>(B.band);
@@ -297,21 +346,41 @@ __global__ __launch_bounds__(TX *TY) void k_iface(IfaceBand B0, IfaceBand B1, in
     }
 }
 
+constexpr int IF_TX = 32, IF_TY = 8;
+
 template <class S, class P>
-static void launch_iface_prec(hipStream_t s, int GR, IfaceBand B0, IfaceBand B1, int Xc, int Yc, HermiteW<P> H,
-                              P r) {
-    constexpr int TX = 32, TY = 8;
-    dim3 grd(((Xc + TX - 1) / TX) * ((Yc + TY - 1) / TY), 2), blk(TX * TY);
-    switch (GR) {
-    case 1: hipLaunchKernelGGL((k_iface<S, P, 1, TX, TY>), grd, blk, 0, s, B0, B1, Xc, Yc, H, r); break;
-    case 2: hipLaunchKernelGGL((k_iface<S, P, 2, TX, TY>), grd, blk, 0, s, B0, B1, Xc, Yc, H, r); break;
-    case 3: hipLaunchKernelGGL((k_iface<S, P, 3, TX, TY>), grd, blk, 0, s, B0, B1, Xc, Yc, H, r); break;
-    default: hipLaunchKernelGGL((k_iface<S, P, 4, TX, TY>), grd, blk, 0, s, B0, B1, Xc, Yc, H, r); break;
+static hipError_t iface_items(hipStream_t s, int GR, const IfaceItem *items, int n, HermiteW<P> H) {
+    for (int i0 = 0; i0 < n; i0 += BATCH_MAX) {
+        Batch<IfaceItem> B;
+        const int blocks = fill_batch(B, items + i0, std::min(BATCH_MAX, n - i0), [](const IfaceItem &a) {
+            return ((a.Xc + IF_TX - 1) / IF_TX) * ((a.Yc + IF_TY - 1) / IF_TY);
+        });
+        const dim3 grd(blocks), blk(IF_TX * IF_TY);
+        switch (GR) {
+        case 1: hipLaunchKernelGGL((k_iface<S, P, 1, IF_TX, IF_TY>), grd, blk, 0, s, B, H); break;
+        case 2: hipLaunchKernelGGL((k_iface<S, P, 2, IF_TX, IF_TY>), grd, blk, 0, s, B, H); break;
+        case 3: hipLaunchKernelGGL((k_iface<S, P, 3, IF_TX, IF_TY>), grd, blk, 0, s, B, H); break;
+        default: hipLaunchKernelGGL((k_iface<S, P, 4, IF_TX, IF_TY>), grd, blk, 0, s, B, H); break;
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
+    return hipSuccess;
 }
 
-hipError_t launch_interface(hipStream_t s, dnsg_ctx *h, int iter) {
-    const int Xc = h->d.Xc, Yc = h->d.Yc, GR = h->d.GR;
+// both bands of one handle: lower writes its top stored layer, upper its layer 0
+static void iface_pair(const dnsg_ctx *h, IfaceItem *out) {
+    // tau_f/(GR tau_c) written with the reference's variables (omega = 1/tau):
+    // tauc/((double)GR*tauf) (ctfdtransfer_opt.c:93)
+    const double rescale = h->d.omega_c / ((double)h->d.GR * h->d.omega_f);
+    const int zds = h->upper.nz;
+    out[0] = {h->lev_lo[0], h->lev_lo[1], h->lev_lo[2], h->lower.n(), zds - 1, h->d.Xc, h->d.Yc, rescale};
+    out[1] = {h->lev_up[0], h->lev_up[1], h->lev_up[2], h->upper.n(), 0, h->d.Xc, h->d.Yc, rescale};
+}
+
+hipError_t launch_interface_many(hipStream_t s, dnsg_ctx *const *hs, int n, int iter) {
+    if (n < 1) return hipSuccess;
+    const int GR = hs[0]->d.GR, prec = hs[0]->p.precision;
     // Hermite weights with the reference's expression order (tinterpolate_hrmt.c:20,34)
     const double tmp = (((double)iter) / ((double)GR));
     HermiteW<double> H;
@@ -319,41 +388,43 @@ hipError_t launch_interface(hipStream_t s, dnsg_ctx *h, int iter) {
     H.h10 = (tmp * tmp * tmp - 2. * tmp * tmp + tmp);
     H.h01 = (-2. * tmp * tmp * tmp + 3. * tmp * tmp);
     H.h11 = (tmp * tmp * tmp - tmp * tmp);
-    // tau_f/(GR tau_c) written with the reference's variables (omega = 1/tau):
-    // tauc/((double)GR*tauf) (ctfdtransfer_opt.c:93)
-    const double rescale = h->d.omega_c / ((double)GR * h->d.omega_f);
-    const int zds = h->upper.nz;
-    IfaceBand B0 = {h->lev_lo[0], h->lev_lo[1], h->lev_lo[2], h->lower.n(), zds - 1};  // lower: top stored layer
-    IfaceBand B1 = {h->lev_up[0], h->lev_up[1], h->lev_up[2], h->upper.n(), 0};        // upper: stored layer 0
-    if (h->p.precision == DNSG_PREC_FP64) {
-        launch_iface_prec<double, double>(s, GR, B0, B1, Xc, Yc, H, rescale);
-    } else {
-        HermiteW<float> Hf = {(float)H.h00, (float)H.h10, (float)H.h01, (float)H.h11};
-        if (h->p.precision == DNSG_PREC_FP32) launch_iface_prec<float, float>(s, GR, B0, B1, Xc, Yc, Hf, (float)rescale);
-        else launch_iface_prec<__half, float>(s, GR, B0, B1, Xc, Yc, Hf, (float)rescale);
-    }
-    return hipGetLastError();
+    std::vector<IfaceItem> v(2 * (size_t)n);
+    for (int i = 0; i < n; i++) iface_pair(hs[i], &v[2 * (size_t)i]);
+    if (prec == DNSG_PREC_FP64) return iface_items<double, double>(s, GR, v.data(), 2 * n, H);
+    HermiteW<float> Hf = {(float)H.h00, (float)H.h10, (float)H.h01, (float)H.h11};
+    if (prec == DNSG_PREC_FP32) return iface_items<float, float>(s, GR, v.data(), 2 * n, Hf);
+    return iface_items<__half, float>(s, GR, v.data(), 2 * n, Hf);
 }
 
+hipError_t launch_interface(hipStream_t s, dnsg_ctx *h, int iter) { return launch_interface_many(s, &h, 1, iter); }
+
 // ---- fine -> coarse injection --------------------------------------------------------
-struct FtcdSide {
-    const void *band;  // band post-stream lattice
-    int layer;         // stored layer coincident with the coarse end layer
-    int kc;            // coarse layer written
+struct FtcdItem {
+    const void *band[2];  // lower / upper band post-stream lattice
+    int layer[2];         // stored layer coincident with the coarse end layer
+    int kc[2];            // coarse layer written
+    void *coarse;         // coarse post-stream lattice
+    int Xc, Yc, GR;
+    double fac;           // GR tau_c / tau_f
 };
 
+// item i: workgroups [0, nb) inject the lower band, [nb, 2 nb) the upper band
 template <class S>
-__global__ __launch_bounds__(256) void k_ftcd(FtcdSide A0, FtcdSide A1, void *coarse, int Xc, int Yc, int zdc,
-                                              int GR, int nzb, double fac) {
-    const FtcdSide &A = blockIdx.y == 0 ? A0 : A1;
-    const int n = Xc * Yc;
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_ftcd(Batch<FtcdItem> B) {
+    const int it = batch_find(B.first, B.n, blockIdx.x);
+    const FtcdItem &A = B.it[it];
+    const int Xc = A.Xc, Yc = A.Yc, GR = A.GR;
+    const int n = Xc * Yc, nb = (n + 255) / 256;
+    const int lb = blockIdx.x - B.first[it];
+    const int side = lb >= nb ? 1 : 0;
+    const int id = (lb - side * nb) * 256 + threadIdx.x;
     if (id >= n) return;
     const int ic = id % Xc, jc = id / Xc;
     const int Xf = GR * Xc, Yf = GR * Yc;
-    const S *band = static_cast<const S *>(A.band);
-    S *cs = static_cast<S *>(coarse);
-    const size_t srow = (size_t)A.layer * Yf + (size_t)GR * jc, drow = (size_t)A.kc * Yc + jc;
+    const double fac = A.fac;
+    const S *band = static_cast<const S *>(A.band[side]);
+    S *cs = static_cast<S *>(A.coarse);
+    const size_t srow = (size_t)A.layer[side] * Yf + (size_t)GR * jc, drow = (size_t)A.kc[side] * Yc + jc;
     double f[Q];
     unroll<0, Q>([&](auto qq) {
         constexpr int q = decltype(qq)::value;
@@ -381,23 +452,34 @@ __global__ __launch_bounds__(256) void k_ftcd(FtcdSide A0, FtcdSide A1, void *co
     }
 }
 
-hipError_t launch_ftcd(hipStream_t s, dnsg_ctx *h) {
-    const int Xc = h->d.Xc, Yc = h->d.Yc, GR = h->d.GR, zdc = h->d.zdc, zdf = h->d.zdf;
-    const int nzb = h->lower.nz;
-    // lower band k = zdf-1-GR is stored layer zdf-2-GR; upper band k = GR is stored layer GR
-    FtcdSide A0 = {h->lower.n(), zdf - 2 - GR, 0};
-    FtcdSide A1 = {h->upper.n(), GR, zdc - 1};
-    // GR*(tauf/tauc) with omega = 1/tau (ftcdtransfer.c:66)
-    const double fac = GR * (h->d.omega_f / h->d.omega_c);
-    dim3 grd((Xc * Yc + 255) / 256, 2), blk(256);
-    void *cn = h->coarse.n();
-    if (h->p.precision == DNSG_PREC_FP64)
-        hipLaunchKernelGGL(k_ftcd<double>, grd, blk, 0, s, A0, A1, cn, Xc, Yc, zdc, GR, nzb, fac);
-    else if (h->p.precision == DNSG_PREC_FP32)
-        hipLaunchKernelGGL(k_ftcd<float>, grd, blk, 0, s, A0, A1, cn, Xc, Yc, zdc, GR, nzb, fac);
-    else
-        hipLaunchKernelGGL(k_ftcd<__half>, grd, blk, 0, s, A0, A1, cn, Xc, Yc, zdc, GR, nzb, fac);
-    return hipGetLastError();
+hipError_t launch_ftcd_many(hipStream_t s, dnsg_ctx *const *hs, int n) {
+    if (n < 1) return hipSuccess;
+    std::vector<FtcdItem> v(n);
+    for (int i = 0; i < n; i++) {
+        const dnsg_ctx *h = hs[i];
+        const int zdf = h->d.zdf, GR = h->d.GR;
+        // lower band k = zdf-1-GR is stored layer zdf-2-GR; upper band k = GR is stored layer GR;
+        // GR*(tauf/tauc) with omega = 1/tau (ftcdtransfer.c:66)
+        v[i] = {{h->lower.n(), h->upper.n()}, {zdf - 2 - GR, GR}, {0, h->d.zdc - 1}, h->coarse.n(),
+                h->d.Xc, h->d.Yc, GR, GR * (h->d.omega_f / h->d.omega_c)};
+    }
+    const int prec = hs[0]->p.precision;
+    for (int i0 = 0; i0 < n; i0 += BATCH_MAX) {
+        Batch<FtcdItem> B;
+        const int blocks = fill_batch(B, v.data() + i0, std::min(BATCH_MAX, n - i0),
+                                      [](const FtcdItem &a) { return 2 * ((a.Xc * a.Yc + 255) / 256); });
+        if (prec == DNSG_PREC_FP64)
+            hipLaunchKernelGGL(k_ftcd<double>, dim3(blocks), dim3(256), 0, s, B);
+        else if (prec == DNSG_PREC_FP32)
+            hipLaunchKernelGGL(k_ftcd<float>, dim3(blocks), dim3(256), 0, s, B);
+        else
+            hipLaunchKernelGGL(k_ftcd<__half>, dim3(blocks), dim3(256), 0, s, B);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
+
+hipError_t launch_ftcd(hipStream_t s, dnsg_ctx *h) { return launch_ftcd_many(s, &h, 1); }
 
 }  // namespace dnsg

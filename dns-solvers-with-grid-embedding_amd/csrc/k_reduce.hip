@@ -146,33 +146,64 @@ static RedSet make_set(dnsg_ctx *h, const double *means_c, const double *means_l
 // ---- encalc2 ---------------------------------------------------------------------------
 struct EncArgs {
     const double *pc, *pl, *pu;
-    int st;  // doubles per row partial: [0] sum rho, [1] sum rho u_x
+    int st;            // doubles per partial: [0] sum rho, [1] sum rho u_x
+    int cpr_c, cpr_f;  // partials per row (coarse, bands)
     int Yc, zdc, Yf, zds, GR, zdf, NZs, Xf, force_mode;
     double ub, g_const;
     double *force, *diag;
 };
 
-__global__ __launch_bounds__(RB) void k_fin_encalc2(EncArgs A) {
+// encalc2 partials as one index space: [lower entries | coarse entries | upper entries]
+struct EncSpace {
+    int nl, nc, nu0, nu;  // entries in each part; nu0 = first upper entry used
+    __device__ int total() const { return nl + nc + nu; }
+};
+__device__ __forceinline__ EncSpace enc_space(const EncArgs &A) {
+    EncSpace S;
+    S.nl = A.Yf * (A.zdf - A.GR - 1) * A.cpr_f;  // lower band k = 1 .. zdf-GR-1 (stored m = 0 .. zdf-GR-2)
+    S.nc = A.Yc * A.zdc * A.cpr_c;                // every coarse layer
+    S.nu0 = A.Yf * A.GR * A.cpr_f;                // upper band k = GR .. zdf-2 (stored m = GR .. zds-1)
+    S.nu = A.Yf * A.zds * A.cpr_f - S.nu0;
+    return S;
+}
+
+// v += weighted (momentum, density, coarse momentum, coarse density) of entry e
+__device__ __forceinline__ void enc_entry(const EncArgs &A, const EncSpace &S, int e, double *v) {
+    if (e < S.nl) {
+        v[0] += A.pl[(size_t)e * A.st + 1];
+        v[1] += A.pl[(size_t)e * A.st + 0];
+    } else if (e < S.nl + S.nc) {
+        e -= S.nl;
+        const int z = (e / A.cpr_c) / A.Yc;
+        const double c = (z == 0 || z == A.zdc - 1) ? 0.5 * A.GR * A.GR * A.GR : 1. * A.GR * A.GR * A.GR;
+        const double m = c * A.pc[(size_t)e * A.st + 1], d = c * A.pc[(size_t)e * A.st + 0];
+        v[0] += m; v[1] += d; v[2] += m; v[3] += d;
+    } else {
+        e = e - S.nl - S.nc + S.nu0;
+        const double c = ((e / A.cpr_f) / A.Yf == A.GR) ? 0.5 : 1.;  // half weight at k = GR
+        v[0] += c * A.pu[(size_t)e * A.st + 1];
+        v[1] += c * A.pu[(size_t)e * A.st + 0];
+    }
+}
+
+constexpr int ENC_SEG = 4096;  // entries per first-stage workgroup
+
+__global__ __launch_bounds__(RB) void k_enc_seg(EncArgs A, double *seg) {
+    __shared__ double lds[4][RB / 64];
+    const EncSpace S = enc_space(A);
+    const int e0 = blockIdx.x * ENC_SEG, e1 = min(S.total(), e0 + ENC_SEG);
+    double v[4] = {0., 0., 0., 0.};
+    for (int e = e0 + threadIdx.x; e < e1; e += RB) enc_entry(A, S, e, v);
+    block_sum<4>(v, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; k++) seg[blockIdx.x * 4 + k] = v[k];
+}
+
+__global__ __launch_bounds__(RB) void k_fin_encalc2(EncArgs A, const double *seg, int nseg) {
     __shared__ double lds[4][RB / 64];
     double v[4] = {0., 0., 0., 0.};  // momentum, denst, cmomentum, cdenst
-    const double g3 = 1. * A.GR * A.GR * A.GR, g3h = 0.5 * A.GR * A.GR * A.GR;
-    // lower band k = 1 .. zdf-GR-1  <->  stored m = 0 .. zdf-GR-2, weight 1
-    const int nl = A.Yf * (A.zdf - A.GR - 1);
-    for (int r = threadIdx.x; r < nl; r += RB) { v[0] += A.pl[r * A.st + 1]; v[1] += A.pl[r * A.st + 0]; }
-    // coarse: every layer, GR^3 (half at k = 0, zdc-1)
-    const int nc = A.Yc * A.zdc;
-    for (int r = threadIdx.x; r < nc; r += RB) {
-        const int z = r / A.Yc;
-        const double c = (z == 0 || z == A.zdc - 1) ? g3h : g3;
-        v[0] += c * A.pc[r * A.st + 1]; v[1] += c * A.pc[r * A.st + 0];
-        v[2] += c * A.pc[r * A.st + 1]; v[3] += c * A.pc[r * A.st + 0];
-    }
-    // upper band k = GR .. zdf-2  <->  stored m = GR .. zds-1, half at k = GR
-    const int nu0 = A.Yf * A.GR, nu1 = A.Yf * A.zds;
-    for (int r = nu0 + threadIdx.x; r < nu1; r += RB) {
-        const double c = (r / A.Yf == A.GR) ? 0.5 : 1.;
-        v[0] += c * A.pu[r * A.st + 1]; v[1] += c * A.pu[r * A.st + 0];
-    }
+    for (int b = threadIdx.x; b < nseg; b += RB)
+        for (int k = 0; k < 4; k++) v[k] += seg[b * 4 + k];
     block_sum<4>(v, lds);
     if (threadIdx.x == 0) {
         const double vol = ((double)((A.Xf + 1 - 1.) * (A.Yf + 1 - 1.) * (A.NZs - 2.)));  // encalc2.c:109
@@ -190,10 +221,26 @@ __global__ __launch_bounds__(RB) void k_fin_encalc2(EncArgs A) {
     }
 }
 
-static EncArgs enc_args(dnsg_ctx *h, const double *pc, const double *pl, const double *pu, int st,
+static hipError_t enc_two_stage(hipStream_t s, const EncArgs &A, double *seg) {
+    const int total = A.Yf * (A.zdf - A.GR - 1) * A.cpr_f + A.Yc * A.zdc * A.cpr_c + A.Yf * (A.zds - A.GR) * A.cpr_f;
+    const int nseg = (total + ENC_SEG - 1) / ENC_SEG;
+    hipLaunchKernelGGL(k_enc_seg, dim3(nseg), dim3(RB), 0, s, A, seg);
+    hipLaunchKernelGGL(k_fin_encalc2, dim3(1), dim3(RB), 0, s, A, (const double *)seg, nseg);
+    return hipGetLastError();
+}
+
+size_t enc_seg_doubles(const dnsg_ctx *h) {
+    const size_t total = (size_t)h->d.Yc * h->d.zdc * cs_chunks(h->d.Xc) +
+                         2 * (size_t)h->d.Yf * (h->d.zdf - 1) * cs_chunks(h->d.Xf);
+    return 4 * (total / ENC_SEG + 2);
+}
+
+static EncArgs enc_args(dnsg_ctx *h, const double *pc, const double *pl, const double *pu, int st, bool fused,
                         double *diag_rec) {
     EncArgs A;
     A.pc = pc; A.pl = pl; A.pu = pu; A.st = st;
+    A.cpr_c = fused ? cs_chunks(h->coarse.nx) : 1;
+    A.cpr_f = fused ? cs_chunks(h->lower.nx) : 1;
     A.Yc = h->d.Yc; A.zdc = h->d.zdc; A.Yf = h->d.Yf; A.zds = h->lower.nz; A.GR = h->d.GR; A.zdf = h->d.zdf;
     A.NZs = h->d.NZs; A.Xf = h->d.Xf; A.force_mode = h->p.force_mode; A.ub = h->p.ub; A.g_const = h->p.g_const;
     A.force = h->d_force; A.diag = diag_rec;
@@ -202,19 +249,57 @@ static EncArgs enc_args(dnsg_ctx *h, const double *pc, const double *pl, const d
 
 // encalc2 from the row sums the collide-stream kernels wrote (RED_W per row)
 hipError_t launch_encalc2_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec) {
-    const double *pc = h->red, *pl = pc + (size_t)h->coarse.ny * h->coarse.nz * RED_W;
-    const double *pu = pl + (size_t)h->lower.ny * h->lower.nz * RED_W;
-    EncArgs A = enc_args(h, pc, pl, pu, RED_W, diag_rec);
-    hipLaunchKernelGGL(k_fin_encalc2, dim3(1), dim3(RB), 0, s, A);
-    return hipGetLastError();
+    const double *pc = h->red, *pl = pc + (size_t)h->coarse.ny * h->coarse.nz * cs_chunks(h->coarse.nx) * RED_W;
+    const double *pu = pl + (size_t)h->lower.ny * h->lower.nz * cs_chunks(h->lower.nx) * RED_W;
+    EncArgs A = enc_args(h, pc, pl, pu, RED_W, true, diag_rec);
+    return enc_two_stage(s, A, h->seg);
+}
+
+// Bulk momentum and density of many solves at once: one workgroup per solve sums
+// its whole encalc2 index space in a fixed order (deterministic) and writes
+// <rho u_x>, <rho> to its item's output pair.
+__global__ __launch_bounds__(RB) void k_flux_items(Batch<EncArgs> B) {
+    __shared__ double lds[4][RB / 64];
+    const EncArgs &A = B.it[blockIdx.x];
+    const EncSpace S = enc_space(A);
+    double v[4] = {0., 0., 0., 0.};
+    for (int e = threadIdx.x; e < S.total(); e += RB) enc_entry(A, S, e, v);
+    block_sum<4>(v, lds);
+    if (threadIdx.x == 0) {
+        const double vol = ((double)((A.Xf + 1 - 1.) * (A.Yf + 1 - 1.) * (A.NZs - 2.)));  // encalc2.c:109
+        A.diag[0] = v[0] / vol;
+        A.diag[1] = v[1] / vol;
+    }
+}
+
+static EncArgs enc_args(dnsg_ctx *h, const double *pc, const double *pl, const double *pu, int st, bool fused,
+                        double *diag_rec);
+
+hipError_t launch_flux_many(hipStream_t s, dnsg_ctx *const *hs, int n, double *out, const int *slot) {
+    for (int i0 = 0; i0 < n; i0 += BATCH_MAX) {
+        Batch<EncArgs> B;
+        B.n = std::min(BATCH_MAX, n - i0);
+        for (int i = 0; i < BATCH_MAX; i++) {
+            dnsg_ctx *h = hs[i0 + (i < B.n ? i : 0)];
+            const double *pc = h->red;
+            const double *pl = pc + (size_t)h->coarse.ny * h->coarse.nz * cs_chunks(h->coarse.nx) * RED_W;
+            const double *pu = pl + (size_t)h->lower.ny * h->lower.nz * cs_chunks(h->lower.nx) * RED_W;
+            B.it[i] = enc_args(h, pc, pl, pu, RED_W, true, out + 2 * (size_t)slot[i0 + (i < B.n ? i : 0)]);
+            B.first[i] = i < B.n ? i : B.n;
+        }
+        B.first[BATCH_MAX] = B.n;
+        hipLaunchKernelGGL(k_flux_items, dim3(B.n), dim3(RB), 0, s, B);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_encalc2(hipStream_t s, dnsg_ctx *h, double *diag_rec) {
     RedSet R = make_set(h, nullptr, nullptr, nullptr);
     row_moments<1>(s, h->p.precision, R, 3);
-    EncArgs A = enc_args(h, R.g[0].part, R.g[1].part, R.g[2].part, NMOM, diag_rec);
-    hipLaunchKernelGGL(k_fin_encalc2, dim3(1), dim3(RB), 0, s, A);
-    return hipGetLastError();
+    EncArgs A = enc_args(h, R.g[0].part, R.g[1].part, R.g[2].part, NMOM, false, diag_rec);
+    return enc_two_stage(s, A, h->seg);
 }
 
 
@@ -270,19 +355,18 @@ __global__ __launch_bounds__(RB) void k_fin_wshear(WsArgs A) {
 
 // wall stress from the row sums of the first band substep (RED_W per row):
 // sum_x (-2u1 + 3u2 - u3) = -2 S1 + 3 S2 - S3 with S_m = sum_x u_x of layer m
-__global__ __launch_bounds__(RB) void k_fin_wshear_fused(WsArgs A, const double *pl, const double *pu, int nz) {
+__global__ __launch_bounds__(RB) void k_fin_wshear_fused(WsArgs A, const double *pl, const double *pu, int nz,
+                                                         int cpr) {
     __shared__ double lds[4][RB / 64];
     double v[4] = {0., 0., 0., 0.};
     const int Yf = A.Yf;
-    for (int y = threadIdx.x; y < Yf; y += RB) {
-        const double *l0 = pl + ((size_t)0 * Yf + y) * RED_W, *l1 = pl + ((size_t)1 * Yf + y) * RED_W;
-        const double *l2 = pl + ((size_t)2 * Yf + y) * RED_W;
-        const double *u1 = pu + ((size_t)(nz - 1) * Yf + y) * RED_W, *u2 = pu + ((size_t)(nz - 2) * Yf + y) * RED_W;
-        const double *u3 = pu + ((size_t)(nz - 3) * Yf + y) * RED_W;
-        v[0] += (-2. * l0[2] + 3. * l1[2] - l2[2]);
-        v[1] += l0[0];
-        v[2] += -(2. * u1[2] - 3. * u2[2] + u3[2]);
-        v[3] += u3[0];
+    auto at = [&](const double *p, int m, int y, int c, int k) { return p[(((size_t)m * Yf + y) * cpr + c) * RED_W + k]; };
+    for (int e = threadIdx.x; e < Yf * cpr; e += RB) {
+        const int y = e / cpr, c = e % cpr;
+        v[0] += (-2. * at(pl, 0, y, c, 2) + 3. * at(pl, 1, y, c, 2) - at(pl, 2, y, c, 2));
+        v[1] += at(pl, 0, y, c, 0);
+        v[2] += -(2. * at(pu, nz - 1, y, c, 2) - 3. * at(pu, nz - 2, y, c, 2) + at(pu, nz - 3, y, c, 2));
+        v[3] += at(pu, nz - 3, y, c, 0);
     }
     block_sum<4>(v, lds);
     if (threadIdx.x == 0) wshear_finish(A, v);
@@ -305,12 +389,13 @@ __device__ void wshear_finish(const WsArgs &A, const double *v) {
 }
 
 hipError_t launch_wshear_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec) {
-    const double *pl = h->red + (size_t)h->coarse.ny * h->coarse.nz * RED_W;
-    const double *pu = pl + (size_t)h->lower.ny * h->lower.nz * RED_W;
+    const int cpr = cs_chunks(h->lower.nx);
+    const double *pl = h->red + (size_t)h->coarse.ny * h->coarse.nz * cs_chunks(h->coarse.nx) * RED_W;
+    const double *pu = pl + (size_t)h->lower.ny * h->lower.nz * cpr * RED_W;
     WsArgs A;
     A.part = nullptr; A.Xf = h->d.Xf; A.Yf = h->d.Yf; A.NZs = h->d.NZs; A.GR = h->d.GR; A.omega_f = h->d.omega_f;
     A.force = h->d_force; A.diag = diag_rec; A.out = nullptr;
-    hipLaunchKernelGGL(k_fin_wshear_fused, dim3(1), dim3(RB), 0, s, A, pl, pu, h->lower.nz);
+    hipLaunchKernelGGL(k_fin_wshear_fused, dim3(1), dim3(RB), 0, s, A, pl, pu, h->lower.nz, cpr);
     return hipGetLastError();
 }
 

@@ -1,10 +1,9 @@
 // k_stream.hip -- fused BGK collide + push-stream on the coarse grid and the
 // wall bands, the initial-state kernel and the AoS <-> SoA transposes.
 //
-// One workgroup owns one (y, z) row of a lattice and walks it in x, so every
-// write of the row -- including the periodic wrap x = nx-1 -> 0 -- lands in
-// cache lines that only this workgroup touches.  Reads are coalesced per
-// direction plane; writes are the same rows shifted by e_x.
+// One workgroup owns a chunk of CS_THREADS nodes of one (y, z) row: its reads
+// are 19 contiguous vectors of the row block, its writes the same vectors of
+// the 9 neighbouring rows shifted by e_x.
 //
 // Boundary rules at the stored z ends (xfcomputations.c:215-442):
 //   BOUND_DROP  populations leaving the lattice are not written (the bounded
@@ -31,22 +30,31 @@ struct ComputeOf<double> {
 __device__ __forceinline__ void collide_any(double *f, double G, double om) { collide64(f, G, om); }
 __device__ __forceinline__ void collide_any(float *f, float G, float om) { collide32(f, G, om); }
 
-struct LatPair {
-    LatArgs l[2];
-};
-
 __device__ __forceinline__ double wave_sum64(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     return v;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs; remap the linear block
+// id so that each XCD gets one contiguous range of (row, chunk) pairs.  The
+// neighbouring chunks of a row, whose e_x-shifted writes share cache lines,
+// then meet in the same L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    const int per = nb >> 3;
+    return (b < (per << 3)) ? (b & 7) * per + (b >> 3) : b;
+}
+
 template <class S>
-__global__ __launch_bounds__(256) void k_collide_stream(LatPair P, const double *__restrict__ force) {
-    const LatArgs &L = P.l[blockIdx.y];
+__global__ __launch_bounds__(CS_THREADS) void k_collide_stream(Batch<LatArgs> P) {
+    const int bg = xcd_remap(blockIdx.x, gridDim.x);
+    const int it = batch_find(P.first, P.n, bg);
+    const LatArgs &L = P.it[it];
     const int nx = L.nx, ny = L.ny, nz = L.nz;
-    const int row = blockIdx.x;
-    if (row >= ny * nz) return;
+    const int cpr = (nx + CS_THREADS - 1) / CS_THREADS;  // chunks per row
+    const int b = bg - P.first[it];
+    const int row = b / cpr, chunk = b % cpr;
+    const double *__restrict__ force = L.force;
     using T = typename ComputeOf<S>::T;
     const int y = row % ny, z = row / ny;
     const S *__restrict__ cur = static_cast<const S *>(L.cur) + (size_t)row * Q * nx;
@@ -57,7 +65,8 @@ __global__ __launch_bounds__(256) void k_collide_stream(LatPair P, const double 
     const int ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
     const bool red = L.red != nullptr;
     double r_rho = 0., r_mom = 0., r_u = 0.;
-    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+    const int x = chunk * CS_THREADS + threadIdx.x;
+    if (x < nx) {  // a partial last chunk still joins the reduction below
         T f[Q];
         unroll<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
@@ -113,28 +122,38 @@ __global__ __launch_bounds__(256) void k_collide_stream(LatPair P, const double 
         __syncthreads();
         if (threadIdx.x == 0) {
             const int nw = (blockDim.x + 63) >> 6;
-            double a = 0., b = 0., c = 0.;
-            for (int w = 0; w < nw; w++) { a += lds[0][w]; b += lds[1][w]; c += lds[2][w]; }
-            L.red[(size_t)row * RED_W + 0] = a;
-            L.red[(size_t)row * RED_W + 1] = b;
-            L.red[(size_t)row * RED_W + 2] = c;
+            double a = 0., bb = 0., c = 0.;
+            for (int w = 0; w < nw; w++) { a += lds[0][w]; bb += lds[1][w]; c += lds[2][w]; }
+            const size_t e = ((size_t)row * cpr + chunk) * RED_W;
+            L.red[e + 0] = a;
+            L.red[e + 1] = bb;
+            L.red[e + 2] = c;
         }
     }
 }
 
-hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, const double *force, int prec) {
-    LatPair P;
-    int rows = 0;
-    for (int i = 0; i < 2; i++) P.l[i] = lats[i < nlat ? i : 0];
-    for (int i = 0; i < nlat; i++) rows = (lats[i].ny * lats[i].nz > rows) ? lats[i].ny * lats[i].nz : rows;
-    dim3 grid(rows, nlat), block(256);
-    if (prec == DNSG_PREC_FP64)
-        hipLaunchKernelGGL(k_collide_stream<double>, grid, block, 0, s, P, force);
-    else if (prec == DNSG_PREC_FP32)
-        hipLaunchKernelGGL(k_collide_stream<float>, grid, block, 0, s, P, force);
-    else
-        hipLaunchKernelGGL(k_collide_stream<__half>, grid, block, 0, s, P, force);
-    return hipGetLastError();
+hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, int prec) {
+    for (int i0 = 0; i0 < nlat; i0 += BATCH_MAX) {
+        Batch<LatArgs> P;
+        P.n = nlat - i0 < BATCH_MAX ? nlat - i0 : BATCH_MAX;
+        int blocks = 0;
+        for (int i = 0; i < BATCH_MAX; i++) {
+            P.it[i] = lats[i0 + (i < P.n ? i : 0)];
+            P.first[i] = blocks;
+            if (i < P.n) blocks += P.it[i].ny * P.it[i].nz * cs_chunks(P.it[i].nx);
+        }
+        P.first[BATCH_MAX] = blocks;
+        if (blocks == 0) continue;
+        if (prec == DNSG_PREC_FP64)
+            hipLaunchKernelGGL(k_collide_stream<double>, dim3(blocks), dim3(CS_THREADS), 0, s, P);
+        else if (prec == DNSG_PREC_FP32)
+            hipLaunchKernelGGL(k_collide_stream<float>, dim3(blocks), dim3(CS_THREADS), 0, s, P);
+        else
+            hipLaunchKernelGGL(k_collide_stream<__half>, dim3(blocks), dim3(CS_THREADS), 0, s, P);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // ---- initial state --------------------------------------------------------------

@@ -245,6 +245,65 @@ int dnsg_stats_reset(dnsg_t *h);
  * 1 = lower band, 2 = upper band; vel is [X][Y][Z][3], den is [X][Y][Z]. */
 int dnsg_velocity_snapshot(dnsg_t *h, int grid, double *vel, double *den);
 
+/* ---- serving queue: steady slip-length solves, continuously batched -----------
+ * No counterpart in the reference, which runs one channel per MPI job.  It is
+ * for parametric sweeps: many independent channels of uneven sizes, each driven
+ * by a constant body force (force_mode = DNSG_FORCE_CONST) until its bulk flux
+ * is steady, each returning its effective slip length.
+ *
+ * Requests wait in arrival order and are admitted, first fit, while the
+ * resident solves' device bytes stay within the budget and fewer than
+ * max_active are resident.  All resident solves of one precision advance
+ * together: every kernel of the step runs as one launch over up to 16 solves
+ * (collide-stream, interface sampling, interface fill per gratio, injection,
+ * flux sums).  Every check_every steps the bulk velocities of all resident
+ * solves are read back in one copy; a solve whose bulk velocity changed by at
+ * most tol (relative) over the last `window` steps retires, frees its memory
+ * and makes room for the next admissions.  One host thread per queue. */
+typedef struct dnsg_queue dnsg_queue_t;
+
+#define DNSG_QUEUE_MAX_ACTIVE 256
+
+typedef struct dnsg_queue_opts {
+    size_t hbm_budget;  /* device bytes the resident solves may hold; 0 = free HBM at creation minus 4 GiB */
+    int max_active;     /* resident solves at most, 1..DNSG_QUEUE_MAX_ACTIVE */
+    int check_every;    /* steps between flux reads; divides window */
+    int window;         /* convergence window in coarse steps */
+    double tol;         /* converged when |U(t) - U(t - window)| <= tol |U(t)| */
+    long max_steps;     /* a solve still moving after this many steps returns with status 1 */
+} dnsg_queue_opts;
+
+typedef struct dnsg_slip_result {
+    uint64_t id;
+    int status;         /* 0 converged, 1 step cap reached, < 0 a dnsg error code */
+    int precision;
+    long steps;         /* coarse steps taken */
+    double ubulk;       /* <rho u_x> / <rho> over the channel (encalc2.c:109 weights) */
+    double u_noslip;    /* G H^2 / (12 nu_c), H = (NZs-2)/GR: the same channel with no-slip walls */
+    double slip_length; /* b = (H/6)(ubulk/u_noslip - 1), coarse cells; equal slip on both walls */
+    double t_arrival, t_admit, t_done; /* seconds on the queue clock */
+} dnsg_slip_result;
+
+/* window 1000, check_every 100, tol 1e-9, max_steps 200000, max_active 64, budget 0 */
+int dnsg_queue_opts_default(dnsg_queue_opts *o);
+int dnsg_queue_create(const dnsg_queue_opts *o, dnsg_queue_t **out);
+int dnsg_queue_destroy(dnsg_queue_t *q);
+const char *dnsg_queue_last_error(const dnsg_queue_t *q);
+/* seconds since the queue was created */
+double dnsg_queue_clock(const dnsg_queue_t *q);
+/* device bytes one solve holds while resident (0 if the parameters are invalid) */
+size_t dnsg_solve_bytes(const dnsg_params *p);
+/* queue one solve; t_arrival < 0 stamps the current queue clock.  Rejects invalid
+ * parameters, force_mode other than DNSG_FORCE_CONST, and a solve larger than
+ * the whole budget, with DNSG_ERR_ARG. */
+int dnsg_queue_submit(dnsg_queue_t *q, const dnsg_params *p, uint64_t id, double t_arrival);
+/* Admit what fits, advance the resident solves by `steps` coarse steps (rounded
+ * up to a multiple of check_every; returns early when nothing is resident or
+ * waiting), and return up to cap finished solves in done (*ndone of them).
+ * Finished solves beyond cap are kept for the next call. */
+int dnsg_queue_advance(dnsg_queue_t *q, int steps, dnsg_slip_result *done, int cap, int *ndone);
+int dnsg_queue_info(const dnsg_queue_t *q, int *pending, int *active, size_t *resident_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,13 +23,16 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 25
     for n in names:
         assert hasattr(lib, n), f"{n} declared in dnsg.h but not exported"
-    assert set(names) - {"dnsg_last_error"} == set(dnsg.SIGNATURES)
+    special = {"dnsg_last_error", "dnsg_queue_last_error", "dnsg_queue_clock", "dnsg_solve_bytes"}  # non-int returns
+    assert set(names) - special == set(dnsg.SIGNATURES)
 
 
 def test_params_struct_matches_header_layout():
     # field order/size of the ctypes mirror must match the C struct
     assert ctypes.sizeof(dnsg.Params) == 144
     assert ctypes.sizeof(dnsg.Diag) == 9 * 8
+    assert ctypes.sizeof(dnsg.QueueOpts) == 40
+    assert ctypes.sizeof(dnsg.SlipResult) == 72
 
 
 CASES = [

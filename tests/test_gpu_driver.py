@@ -1,0 +1,113 @@
+"""The kept C driver (dnsg-run) end to end against the oracle's driver loop.
+
+Both programs start from the same '-p' restart files (written by the oracle),
+run the reference driver loop (dns-q-mgrd-all-patterns-xyc.c:1254-1375) and
+write the reference's output files into their own directories.  Every file the
+oracle writes must exist in the driver's directory too.
+
+* constant body force: the lattice state (the ftemp half of lbf-c/slf/suf),
+  the interface planes and force.NNNN are bit-equal; reductions (logs,
+  statistics, velocity snapshots, the output_twoD2 profile) agree to 1e-10.
+* flux-controlled force (the reference's own mode): every number agrees to
+  1e-11 relative after 50 steps (the force comes from a sum taken in another
+  order).
+* '-s' rest start: the same comparison from the reference's rest state.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ORC_CLI
+from helpers import PKG_DIR
+
+DRIVER = os.path.join(PKG_DIR, "build", "dnsg-run")
+UB = 0.0333333333333
+MICRO = dict(xdv=16, ydv=8, zdv=12, gratio=2, zds=4, Rebs="20.0", ub=repr(UB), CFL="5.0", ITER=50,
+             pattern=1, gyplussize=4, wyplussize=4)
+SEEDED = dict(init_kind=1, seed=42, u_max="0.05", noise="0.05")
+LOGS = ("flowrate.txt", "density.txt", "str-up.txt", "str-do.txt", "w-str.txt", "p-grad.txt")
+
+
+def _args(kw):
+    return [f"{k}={v}" for k, v in kw.items()]
+
+
+def _run_pair(tmp_path, kw, mode):
+    odir, gdir = tmp_path / "oracle", tmp_path / "driver"
+    odir.mkdir()
+    gdir.mkdir()
+    subprocess.run([ORC_CLI, "init", str(odir)] + _args(kw), check=True)
+    if mode == "p":
+        for f in os.listdir(odir):
+            shutil.copy(odir / f, gdir / f)
+    subprocess.run([ORC_CLI, "run", str(odir)] + _args(kw) + ["threads=4"], check=True, capture_output=True)
+    print(f"driver: dnsg-run -{mode} {' '.join(_args(kw))}", flush=True)
+    r = subprocess.run([DRIVER, f"-{mode}"] + _args(kw) + ["results_dat=0"], cwd=gdir, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return odir, gdir, r.stdout
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300) if b.size else 0.0
+
+
+def _compare(odir, gdir, exact_state, tol):
+    names = sorted(os.listdir(odir))
+    assert any(n.startswith("lbf-c.") for n in names)
+    for name in names:
+        assert (gdir / name).exists(), f"driver did not write {name}"
+        if name.endswith(".txt"):
+            a = np.loadtxt(gdir / name, ndmin=2)
+            b = np.loadtxt(odir / name, ndmin=2)
+            assert a.shape == b.shape, name
+            assert np.array_equal(a[:, 0], b[:, 0]), name
+            assert np.all(np.abs(a[:, 1] - b[:, 1]) <= 2e-12 + tol * np.abs(b[:, 1])), name
+            continue
+        a = np.fromfile(gdir / name, dtype=np.float64)
+        b = np.fromfile(odir / name, dtype=np.float64)
+        assert a.shape == b.shape, name
+        if name.startswith(("lbf-c.", "lbf-slf.", "lbf-suf.")):
+            # ftemp (the state) first; the second half is the reference's collision scratch
+            a, b = a[: a.size // 2], b[: b.size // 2]
+        state = name.startswith(("lbf-", "force."))
+        if exact_state and state:
+            assert np.array_equal(a, b), f"{name}: max diff {np.max(np.abs(a - b)):.3e}"
+        elif name.startswith("force."):
+            # G = ub - <rho u> cancels: bound it against ub, as the controller's input
+            assert np.all(np.abs(a - b) <= tol * UB), f"{name}: {a} vs {b}"
+        else:
+            assert _rel(a, b) <= tol, f"{name}: rel {_rel(a, b):.3e}"
+
+
+@pytest.mark.gpu
+def test_driver_constant_force_bitwise(tmp_path, oracle_built):
+    kw = dict(MICRO, **SEEDED, force_mode=1, g_const="2e-5")
+    odir, gdir, out = _run_pair(tmp_path, kw, "p")
+    _compare(odir, gdir, exact_state=True, tol=1e-10)
+    assert "Average Density in the channel at end" in out
+
+
+@pytest.mark.gpu
+def test_driver_flux_mode(tmp_path, oracle_built):
+    kw = dict(MICRO, **SEEDED)
+    odir, gdir, _ = _run_pair(tmp_path, kw, "p")
+    _compare(odir, gdir, exact_state=False, tol=1e-11)
+
+
+@pytest.mark.gpu
+def test_driver_rest_start(tmp_path, oracle_built):
+    kw = dict(MICRO, init_kind=0)
+    odir, gdir, _ = _run_pair(tmp_path, kw, "s")
+    _compare(odir, gdir, exact_state=False, tol=1e-11)
+
+
+def test_driver_rejects_unknown_key(tmp_path):
+    if not os.path.exists(DRIVER):
+        pytest.skip("driver not built")
+    r = subprocess.run([DRIVER, "-s", "no_such_key=1"], cwd=tmp_path, capture_output=True, text=True, timeout=30)
+    assert r.returncode != 0
+    assert "no_such_key" in (r.stdout + r.stderr)

@@ -1,0 +1,141 @@
+"""The serving queue of steady slip-length solves (dnsg_queue_*).
+
+* a solve gives the same bits alone and in a mix of other solves (the batched
+  kernels keep each solve's arithmetic);
+* the queue's bulk velocity equals the single-solve step's (the FP64 plain path,
+  itself bit-equal to the oracle in constant-force mode) to 1e-12;
+* no-slip walls give a slip length near zero; longitudinal stripes give one
+  independent of Re_b and bracketed by Philip's closed form
+  (b = (L/pi) ln sec(pi phi / 2) for a wall far from the other);
+* admission respects the device-memory budget;
+* FP32 slip lengths agree with FP64 ones.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import dnsg
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(xdv=16, ydv=16, zdv=8, pattern="rslip", period=8, gas_fraction=0.5, re_b=10.0)
+
+
+def _req(gratio=2, precision="fp64", **kw):
+    a = dict(SMALL, **kw)
+    return dnsg.slip_request(a["xdv"], a["ydv"], a["zdv"], gratio, a["pattern"], a["period"], a["gas_fraction"],
+                             a["re_b"], precision=precision)
+
+
+def _drain(q, n, steps=1000):
+    out = {}
+    for _ in range(10000):
+        for r in q.advance(steps):
+            out[r["id"]] = r
+        if len(out) == n:
+            return out
+    raise AssertionError("queue did not drain")
+
+
+def test_alone_equals_mix():
+    target = _req(gratio=2, precision="fp32")
+    q = dnsg.Queue(max_active=1, max_steps=60000)
+    q.submit(target, 0)
+    alone = _drain(q, 1)[0]
+    q.close()
+    others = [_req(gratio=1, precision="fp32", xdv=32, period=16, gas_fraction=0.3),
+              _req(gratio=2, precision="fp32", ydv=32, pattern="pslip", gas_fraction=0.7),
+              _req(gratio=2, precision="fp64", zdv=10),
+              _req(gratio=1, precision="fp32", xdv=48, ydv=32, re_b=40.0)]
+    q = dnsg.Queue(max_active=5, max_steps=60000)
+    for i, p in enumerate(others[:2]):
+        q.submit(p, 10 + i)
+    q.advance(300)  # the target joins solves already in flight
+    q.submit(target, 0)
+    for i, p in enumerate(others[2:]):
+        q.submit(p, 20 + i)
+    mix = _drain(q, 5)
+    q.close()
+    assert alone["status"] == 0, alone
+    assert mix[0]["status"] == 0
+    assert mix[0]["steps"] == alone["steps"]
+    assert mix[0]["ubulk"] == alone["ubulk"]
+    assert mix[0]["slip_length"] == alone["slip_length"]
+
+
+def test_queue_matches_single_solve_step():
+    p = _req(gratio=2, precision="fp64")
+    q = dnsg.Queue(max_active=1, check_every=100, window=200, tol=0.0, max_steps=300)
+    q.submit(p, 0)
+    r = _drain(q, 1)[0]
+    q.close()
+    assert r["status"] == 1 and r["steps"] == 300  # tol 0: runs to the cap
+    s = dnsg.Solver(p)
+    s.init_state()
+    d = s.step(300)
+    s.close()
+    # the queue's last check reads the sums taken at the top of step 299
+    u = d["momentum"][299] / d["density"][299]
+    assert abs(r["ubulk"] - u) <= 1e-12 * abs(u)
+
+
+def _philip(L, phi):
+    """slip length of longitudinal stripes, period L, gas fraction phi (Philip 1972)"""
+    return (L / math.pi) * math.log(1.0 / math.cos(math.pi * phi / 2.0))
+
+
+def test_no_slip_and_stripes_closed_forms():
+    q = dnsg.Queue(max_active=4, max_steps=60000)
+    q.submit(_req(gratio=2, pattern="none", zdv=12), 0)
+    # stripes of period 8 fine cells (4 coarse), the channel 7 periods high
+    q.submit(_req(gratio=2, pattern="rslip", period=8, gas_fraction=0.5, zdv=24), 1)
+    q.submit(_req(gratio=2, pattern="rslip", period=8, gas_fraction=0.5, zdv=24, re_b=25.0), 2)
+    res = _drain(q, 3)
+    q.close()
+    for r in res.values():
+        assert r["status"] == 0, r
+    # no-slip walls: the halfway wall of the Poiseuille closed form
+    assert abs(res[0]["slip_length"]) < 0.05
+    # flow along stripes is unidirectional (Stokes for any Re): b does not depend on Re_b
+    b = res[1]["slip_length"]
+    assert abs(res[2]["slip_length"] - b) <= 0.02 * b, (res[2]["slip_length"], b)
+    # Philip's single-wall value brackets it: at a stripe edge the wall rule sends
+    # diagonal links specular whenever either wall cell they cross is gas, which
+    # widens the effective gas strip by up to one fine cell (1/8 of the period)
+    L = 8 / 2.0
+    assert _philip(L, 0.5) <= b <= 1.15 * _philip(L, 0.5 + 1.0 / 8), (b, _philip(L, 0.5), _philip(L, 0.625))
+
+
+def test_admission_respects_budget():
+    reqs = [_req(gratio=2, precision="fp32", xdv=16 + 16 * (i % 2)) for i in range(5)]
+    per = max(dnsg.solve_bytes(p) for p in reqs)
+    q = dnsg.Queue(max_active=8, hbm_budget=2 * per + 1, max_steps=60000)
+    for i, p in enumerate(reqs):
+        q.submit(p, i)
+    peak = 0
+    out = {}
+    while len(out) < 5:
+        for r in q.advance(200):
+            out[r["id"]] = r
+        info = q.info()
+        peak = max(peak, info["active"])
+        assert info["resident_bytes"] <= 2 * per + 1
+    q.close()
+    assert peak <= 2
+    assert all(r["status"] == 0 for r in out.values())
+    assert all(r["t_admit"] >= r["t_arrival"] and r["t_done"] > r["t_admit"] for r in out.values())
+    with pytest.raises(dnsg.DnsgError):
+        q2 = dnsg.Queue(hbm_budget=per // 2)
+        q2.submit(reqs[0], 0)
+
+
+def test_fp32_slip_matches_fp64():
+    q = dnsg.Queue(max_active=4, max_steps=60000)
+    for i, prec in enumerate(("fp64", "fp32")):
+        q.submit(_req(gratio=2, precision=prec, pattern="pslip", gas_fraction=0.6), i)
+    res = _drain(q, 2)
+    q.close()
+    b64, b32 = res[0]["slip_length"], res[1]["slip_length"]
+    assert res[0]["status"] == 0 and res[1]["status"] == 0, res
+    assert abs(b32 - b64) <= 1e-3 * max(1.0, abs(b64)), (b32, b64)
