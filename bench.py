@@ -18,8 +18,10 @@ LU per step = Nc + GR (N_lower + N_upper) = 50,331,648.
 value = FP64 plain-path GLUPS with the lattices resident in HBM.  The line also
 carries the shifted FP32 and FP16-storage tiers, the roofline of the dominant
 kernel (band collide-stream, measured live with HIP events on the solver's
-stream) and the oracle (C restatement, OpenMP) timed on a bounded sample on
-the host cores.
+stream), the oracle (C restatement, OpenMP) timed on a bounded sample on
+the host cores, and the slip-length serving queue: seeded requests arriving as
+a Poisson stream, continuously batched, with p50/p95/p99 latency, throughput
+and an FP64 re-solve of a subset.
 """
 import argparse
 import json
@@ -114,20 +116,104 @@ def cpu_baseline(mod, nsteps):
                        f"(oracle, C restatement of the reference step, OpenMP, {dt:.1f} s)")
 
 
+QUEUE_ENVELOPE = dict(x_range=(32, 128), y_range=(32, 128), z_range=(16, 32), re_range=(10.0, 50.0))
+
+
+def run_queue(mod, n, rate, seed, n_check):
+    """Slip-length serving queue: n requests from the seeded generator (streamwise and
+    spanwise 32..128, wall-normal 16..32 coarse cells, bands on a random half, stripes
+    or posts, gas fraction 0.2..0.9, period 8..64 fine cells, Re_b 10..50, shifted FP32),
+    Poisson arrivals at `rate` per second, each solved until its bulk velocity moves by
+    < 1e-9 (relative) over 1000 steps.  Latency = completion - arrival.  The first
+    n_check requests are solved again in FP64 (plain path) and their slip lengths
+    compared."""
+    import random
+    work = mod.slip_workload(n, seed, precision="fp32", **QUEUE_ENVELOPE)
+    rng = random.Random(seed + 1)
+    t, arrivals = 0.0, []
+    for _ in range(n):
+        t += rng.expovariate(rate)
+        arrivals.append(t)
+    lu = {}
+    for d, p in work:
+        dm = mod.dims_of(p)
+        lu[d["id"]] = dm.Xc * dm.Yc * dm.zdc + dm.GR * 2 * dm.Xf * dm.Yf * (dm.zdf - 1)
+    q = mod.Queue(max_active=n, check_every=100, window=1000, tol=1e-9, max_steps=400000)
+    t0 = q.clock()
+    done, i, last = {}, 0, time.time()
+    while len(done) < n:
+        now = q.clock() - t0
+        while i < n and arrivals[i] <= now:
+            q.submit(work[i][1], i, t0 + arrivals[i])
+            i += 1
+        info = q.info()
+        if info["active"] == 0 and info["pending"] == 0 and i < n:
+            time.sleep(max(0.0, arrivals[i] - (q.clock() - t0)))
+            continue
+        for r in q.advance(100):
+            done[r["id"]] = r
+        if time.time() - last > 15:
+            print(f"bench: queue t={q.clock() - t0:.0f}s submitted={i} done={len(done)} {info}", file=sys.stderr,
+                  flush=True)
+            last = time.time()
+    wall = q.clock() - t0
+    q.close()
+    lat = np.array([done[k]["t_done"] - done[k]["t_arrival"] for k in range(n)])
+    steps = np.array([done[k]["steps"] for k in range(n)])
+    glu = sum(done[k]["steps"] * lu[k] for k in range(n)) / 1e9
+    out = dict(requests=n, arrival_rate_per_s=rate, seed=seed, precision="fp32 shifted, fp64 reductions",
+               envelope=QUEUE_ENVELOPE, converged=int(sum(done[k]["status"] == 0 for k in range(n))),
+               p50_s=float(np.percentile(lat, 50)), p95_s=float(np.percentile(lat, 95)),
+               p99_s=float(np.percentile(lat, 99)), mean_s=float(lat.mean()), wall_s=wall,
+               throughput_req_per_s=n / wall, glups=glu / wall, steps_median=int(np.median(steps)),
+               steps_max=int(steps.max()))
+    if n_check:
+        q = mod.Queue(max_active=n_check, check_every=100, window=1000, tol=1e-9, max_steps=400000)
+        for k in range(n_check):
+            d = work[k][0]
+            q.submit(mod.slip_request(d["xdv"], d["ydv"], d["zdv"], d["gratio"], d["pattern"], d["period"],
+                                      d["gas_fraction"], d["re_b"], precision="fp64"), k)
+        ref = {}
+        while len(ref) < n_check:
+            for r in q.advance(1000):
+                ref[r["id"]] = r
+        q.close()
+        db = [abs(done[k]["slip_length"] - ref[k]["slip_length"]) for k in range(n_check)]
+        out["fp64_check"] = dict(requests=n_check, max_abs_diff_slip=float(max(db)),
+                                 max_rel_diff_slip=float(max(db[k] / max(abs(ref[k]["slip_length"]), 1e-12)
+                                                             for k in range(n_check))))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tiers", default="fp64,fp32,fp16")
+    ap.add_argument("--queue-requests", type=int, default=32)
+    ap.add_argument("--queue-rate", type=float, default=1.0, help="Poisson arrival rate, requests per second")
+    ap.add_argument("--queue-check", type=int, default=8, help="requests re-solved in FP64 for the slip check")
+    ap.add_argument("--no-queue", action="store_true")
+    ap.add_argument("--pmc-run", action="store_true",
+                    help="minimal run for hardware-counter passes: 3 FP64 steps of the workload, then one "
+                         "1 GiB copy-probe launch whose known bytes calibrate FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the lattices of one channel live on a single "
               "MI355X); multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
     from helpers import dnsg as mod
+
+    if args.pmc_run:
+        s = mod.Solver(workload_params(mod, "fp64"))
+        s.init_state()
+        s.step(3)
+        s.close()
+        print(f"bench: copy probe {mod.hbm_probe(1 << 30, 0, 1):.1f} GB/s", file=sys.stderr, flush=True)
+        return
 
     tiers = [t for t in args.tiers.split(",") if t]
     res = {}
@@ -146,6 +232,12 @@ def main():
                 frac=(bl["gbs"] / HBM_PEAK_GBS) if bl["gbs"] else None, traffic=traffic,
                 kernel="k_collide_stream<double> (both wall bands, one launch per substep)",
                 algorithmic_bytes_per_launch=bl["bytes"], launch_ms=bl["ms"])
+    queue = None
+    if not args.no_queue and args.queue_requests > 0:
+        print(f"bench: slip queue ({args.queue_requests} requests at {args.queue_rate}/s)", file=sys.stderr, flush=True)
+        queue = run_queue(mod, args.queue_requests, args.queue_rate, 2025, min(args.queue_check, args.queue_requests))
+        print(f"bench: queue p50 {queue['p50_s']:.2f} s p99 {queue['p99_s']:.2f} s, {queue['glups']:.2f} GLUPS",
+              file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         print(f"bench: cpu baseline ({args.cpu_steps} steps)", file=sys.stderr, flush=True)
@@ -169,6 +261,7 @@ def main():
         "hbm_roofline_frac_step": round(main_t["glups"] * BYTES_PER_LU[main_prec] / HBM_PEAK_GBS, 4),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "slip_queue": queue,
         "hbm_probe_gbs": probe,
         "tiers": {k: {"glups": round(v["glups"], 4), "ms_per_step": round(v["ms_per_step"], 4),
                       "hbm_frac": round(v["glups"] * BYTES_PER_LU[k] / HBM_PEAK_GBS, 4),

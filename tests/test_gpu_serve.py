@@ -5,7 +5,7 @@
 * the queue's bulk velocity equals the single-solve step's (the FP64 plain path,
   itself bit-equal to the oracle in constant-force mode) to 1e-12;
 * no-slip walls give a slip length near zero; longitudinal stripes give one
-  independent of Re_b and bracketed by Philip's closed form
+  that grows with the gas fraction and is bracketed by Philip's closed form
   (b = (L/pi) ln sec(pi phi / 2) for a wall far from the other);
 * admission respects the device-memory budget;
 * FP32 slip lengths agree with FP64 ones.
@@ -88,23 +88,24 @@ def _philip(L, phi):
 def test_no_slip_and_stripes_closed_forms():
     q = dnsg.Queue(max_active=4, max_steps=60000)
     q.submit(_req(gratio=2, pattern="none", zdv=12), 0)
-    # stripes of period 8 fine cells (4 coarse), the channel 7 periods high
-    q.submit(_req(gratio=2, pattern="rslip", period=8, gas_fraction=0.5, zdv=24), 1)
-    q.submit(_req(gratio=2, pattern="rslip", period=8, gas_fraction=0.5, zdv=24, re_b=25.0), 2)
-    res = _drain(q, 3)
+    # stripes of period 8 fine cells (4 coarse) at gas fractions 1/4, 1/2, 3/4; the channel
+    # is 7 periods high and every solve has the same relaxation times
+    for i, phi in enumerate((0.25, 0.5, 0.75)):
+        q.submit(_req(gratio=2, pattern="rslip", period=8, gas_fraction=phi, zdv=24), 1 + i)
+    res = _drain(q, 4)
     q.close()
     for r in res.values():
         assert r["status"] == 0, r
     # no-slip walls: the halfway wall of the Poiseuille closed form
     assert abs(res[0]["slip_length"]) < 0.05
-    # flow along stripes is unidirectional (Stokes for any Re): b does not depend on Re_b
-    b = res[1]["slip_length"]
-    assert abs(res[2]["slip_length"] - b) <= 0.02 * b, (res[2]["slip_length"], b)
-    # Philip's single-wall value brackets it: at a stripe edge the wall rule sends
-    # diagonal links specular whenever either wall cell they cross is gas, which
-    # widens the effective gas strip by up to one fine cell (1/8 of the period)
+    b = [res[i]["slip_length"] for i in (1, 2, 3)]
+    assert 0 < b[0] < b[1] < b[2], b
+    # Philip's single-wall value brackets the half-gas case: at a stripe edge the wall
+    # rule sends diagonal links specular whenever either wall cell they cross is gas,
+    # which widens the effective gas strip by up to one fine cell (1/8 of the period).
+    # (The discrete slip also depends on the relaxation time, as BGK walls do.)
     L = 8 / 2.0
-    assert _philip(L, 0.5) <= b <= 1.15 * _philip(L, 0.5 + 1.0 / 8), (b, _philip(L, 0.5), _philip(L, 0.625))
+    assert _philip(L, 0.5) <= b[1] <= 1.15 * _philip(L, 0.5 + 1.0 / 8), (b[1], _philip(L, 0.5), _philip(L, 0.625))
 
 
 def test_admission_respects_budget():
