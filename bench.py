@@ -116,6 +116,56 @@ def cpu_baseline(mod, nsteps):
                        f"(oracle, C restatement of the reference step, OpenMP, {dt:.1f} s)")
 
 
+def cfg_small_params(mod, prec):
+    """The 64x32x32 perturbed laminar channel: coarse 64 (streamwise) x 32 (spanwise) x 32
+    (wall-normal), factor-2 bands over 4 coarse cells at each wall, tau_c 0.52 (tau_f 0.54),
+    constant body force 1e-6, rho = 1, parabolic profile u_max 0.05 with seeded +-5% noise."""
+    return mod.make_params(xdv=64, ydv=32, zdv=32, gratio=2, zds=8, tau_c=0.52, ub=0.05 * 2 / 3,
+                           force_mode="const", g_const=1e-6, precision=prec, init_kind="laminar_noise",
+                           seed=1234, u_max=0.05, noise=0.05, CFL=1.0)
+
+
+def precision_tier(mod, steps=500):
+    """Shifted FP32 and FP16-storage runs of the 64x32x32 channel against the FP64 plain
+    path (which equals the oracle bit for bit under constant force): mean U+ (each run
+    scaled by its own friction velocity from its wall stress), u', v', w' rms, -u'v'
+    (v wall-normal) over the interior profile slots, the mean slip velocity at the walls
+    (U extrapolated to the halfway wall from the first two fine layers, both walls
+    averaged) and the bulk flux.  Errors are max |tier - fp64| over the profile, also
+    relative to the FP64 profile's largest magnitude."""
+    res, prof = {}, {}
+    for prec in ("fp64", "fp32", "fp16"):
+        s = mod.Solver(cfg_small_params(mod, prec))
+        s.init_state()
+        d = s.step(steps)
+        st = s.statistics()
+        NZ = s.dims.NZ
+        s.close()
+        st = np.asarray(st).reshape(9, NZ)
+        utau = float(np.sqrt(abs(d["stress"][-1])))
+        inner = slice(1, NZ - 1)
+        U = st[0, inner] / utau
+        us = 0.5 * ((1.5 * st[0, 1] - 0.5 * st[0, 2]) + (1.5 * st[0, NZ - 2] - 0.5 * st[0, NZ - 3]))
+        prof[prec] = dict(U_plus=U, u_rms=st[3, inner] / utau, v_rms=st[5, inner] / utau, w_rms=st[4, inner] / utau,
+                          uv=-st[6, inner] / utau ** 2, slip=us, flux=float(d["flowrate"][-1]), utau=utau)
+    ref = prof["fp64"]
+    for prec in ("fp32", "fp16"):
+        t = prof[prec]
+        e = {}
+        for k in ("U_plus", "u_rms", "v_rms", "w_rms", "uv"):
+            a = float(np.max(np.abs(t[k] - ref[k])))
+            e[k] = dict(max_abs=a, rel_to_profile_max=a / max(float(np.max(np.abs(ref[k]))), 1e-300))
+        e["slip_velocity"] = dict(value=t["slip"], fp64=ref["slip"], abs=abs(t["slip"] - ref["slip"]))
+        e["flux_rel"] = abs(t["flux"] - ref["flux"]) / abs(ref["flux"])
+        e["utau_rel"] = abs(t["utau"] - ref["utau"]) / ref["utau"]
+        res[prec] = e
+    res["workload"] = ("64x32x32 coarse, GR 2 bands 128x64x8, tau_c 0.52, g 1e-6, seeded laminar + 5% noise, "
+                       f"{steps} steps; v'rms and -u'v' use the wall-normal component")
+    res["fp64_u_tau"] = ref["utau"]
+    res["fp64_slip_velocity"] = ref["slip"]
+    return res
+
+
 QUEUE_ENVELOPE = dict(x_range=(32, 128), y_range=(32, 128), z_range=(16, 32), re_range=(10.0, 50.0))
 
 
@@ -232,6 +282,8 @@ def main():
                 frac=(bl["gbs"] / HBM_PEAK_GBS) if bl["gbs"] else None, traffic=traffic,
                 kernel="k_collide_stream<double> (both wall bands, one launch per substep)",
                 algorithmic_bytes_per_launch=bl["bytes"], launch_ms=bl["ms"])
+    print("bench: precision tier report (64x32x32 channel, 500 steps)", file=sys.stderr, flush=True)
+    prec_report = precision_tier(mod)
     queue = None
     if not args.no_queue and args.queue_requests > 0:
         print(f"bench: slip queue ({args.queue_requests} requests at {args.queue_rate}/s)", file=sys.stderr, flush=True)
@@ -262,6 +314,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "slip_queue": queue,
+        "precision_tier": prec_report,
         "hbm_probe_gbs": probe,
         "tiers": {k: {"glups": round(v["glups"], 4), "ms_per_step": round(v["ms_per_step"], 4),
                       "hbm_frac": round(v["glups"] * BYTES_PER_LU[k] / HBM_PEAK_GBS, 4),

@@ -31,7 +31,7 @@ def test_shifted_tier_against_fp64_oracle(prec):
     sg, so = s.statistics(), o.statistics()
     assert relmax(sg[0], so[0]) <= TOL[prec], relmax(sg[0], so[0])
     assert relmax(dg["flowrate"], dr["flowrate"]) <= TOL[prec]
-    assert abs(s.mass() - o.mass()) <= 1e-6 if prec == "fp32" else 1e-4
+    assert abs(s.mass() - o.mass()) <= (1e-6 if prec == "fp32" else 1e-4)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp16"])
