@@ -212,7 +212,10 @@ def run_queue(mod, n, rate, seed, n_check):
     steps = np.array([done[k]["steps"] for k in range(n)])
     glu = sum(done[k]["steps"] * lu[k] for k in range(n)) / 1e9
     out = dict(requests=n, arrival_rate_per_s=rate, seed=seed, precision="fp32 shifted, fp64 reductions",
-               envelope=QUEUE_ENVELOPE, converged=int(sum(done[k]["status"] == 0 for k in range(n))),
+               envelope=QUEUE_ENVELOPE, converged_tol=int(sum(done[k]["status"] == 0 for k in range(n))),
+               steady_at_fp32_floor=int(sum(done[k]["status"] == 2 for k in range(n))),
+               capped=int(sum(done[k]["status"] == 1 for k in range(n))),
+               criterion="|dU/U| <= 1e-9 over 1000 steps, or (FP32) within 3e-5 of U 1..5 windows back",
                p50_s=float(np.percentile(lat, 50)), p95_s=float(np.percentile(lat, 95)),
                p99_s=float(np.percentile(lat, 99)), mean_s=float(lat.mean()), wall_s=wall,
                throughput_req_per_s=n / wall, glups=glu / wall, steps_median=int(np.median(steps)),

@@ -69,7 +69,8 @@ _P = ctypes.c_void_p
 _DP = ctypes.POINTER(ctypes.c_double)
 class QueueOpts(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_active", ctypes.c_int), ("check_every", ctypes.c_int),
-                ("window", ctypes.c_int), ("tol", ctypes.c_double), ("max_steps", ctypes.c_long)]
+                ("window", ctypes.c_int), ("tol", ctypes.c_double), ("max_steps", ctypes.c_long),
+                ("floor_tol", ctypes.c_double)]
 
 
 class SlipResult(ctypes.Structure):

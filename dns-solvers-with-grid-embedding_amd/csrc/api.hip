@@ -227,6 +227,9 @@ struct SecTimer {
 // load: they sum it there (per-row partials) instead of re-reading the lattices.
 // Only the flux controller needs G = ub - <rho u> before the coarse collision, so
 // in that mode encalc2 keeps its own reduction pass.
+// The coarse collide-stream and the first band substep touch disjoint lattices
+// and read the same force, so they run as one launch over the three lattices
+// (xcomputations and the first xfcomputations of the reference's loop).
 int do_step(dnsg_t *h, double *rec, SecTimer *T = nullptr) {
     const bool flux = h->p.force_mode == DNSG_FORCE_FLUX;
     if (T) T->mark(h->stream, -1);
@@ -234,28 +237,26 @@ int do_step(dnsg_t *h, double *rec, SecTimer *T = nullptr) {
         HIPCHK(h, launch_encalc2(h->stream, h, rec));
         if (T) T->mark(h->stream, SEC_REDUCE);
     }
-    {
-        LatArgs L = coarse_args(h);
-        if (!flux) L.red = red_coarse(h);
-        HIPCHK(h, launch_collide_stream(h->stream, &L, 1, h->p.precision));
-        if (T) T->mark(h->stream, SEC_COARSE);
-        rotate_levels(h);
-        HIPCHK(h, launch_sample(h->stream, h, h->coarse.n(), h->lev_lo[0], h->lev_up[0]));
-        if (T) T->mark(h->stream, SEC_SAMPLE);
-    }
     for (int iter = 1; iter <= h->d.GR; iter++) {
-        LatArgs L[2];
-        band_args(h, L);
+        LatArgs L[3];
         if (iter == 1) {
-            L[0].red = red_lower(h);
-            L[1].red = red_upper(h);
-        }
-        HIPCHK(h, launch_collide_stream(h->stream, L, 2, h->p.precision));
-        if (T) T->mark(h->stream, SEC_BAND);
-        if (iter == 1) {
+            L[0] = coarse_args(h);
+            band_args(h, &L[1]);
+            if (!flux) L[0].red = red_coarse(h);
+            L[1].red = red_lower(h);
+            L[2].red = red_upper(h);
+            HIPCHK(h, launch_collide_stream(h->stream, L, 3, h->p.precision));
+            if (T) T->mark(h->stream, SEC_COARSE);
+            rotate_levels(h);
+            HIPCHK(h, launch_sample(h->stream, h, h->coarse.n(), h->lev_lo[0], h->lev_up[0]));
+            if (T) T->mark(h->stream, SEC_SAMPLE);
             if (!flux) HIPCHK(h, launch_encalc2_fused(h->stream, h, rec));
             HIPCHK(h, launch_wshear_fused(h->stream, h, rec));
             if (T) T->mark(h->stream, SEC_REDUCE);
+        } else {
+            band_args(h, L);
+            HIPCHK(h, launch_collide_stream(h->stream, L, 2, h->p.precision));
+            if (T) T->mark(h->stream, SEC_BAND);
         }
         HIPCHK(h, launch_interface(h->stream, h, iter));
         if (iter == h->d.GR) HIPCHK(h, launch_ftcd(h->stream, h));

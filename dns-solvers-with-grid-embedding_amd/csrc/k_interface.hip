@@ -24,7 +24,6 @@
 
 namespace dnsg {
 
-constexpr int NV = 23;  // rho, u_x, u_y, u_z, f_neq[19]
 
 // ---- sampling --------------------------------------------------------------------
 struct SampleItem {
@@ -216,6 +215,7 @@ struct IfaceItem {
     void *band;                 // band post-stream lattice
     int layer;                  // stored interface layer
     int Xc, Yc;
+    int groups;                 // workgroups per tile, each filling a share of the 19 directions
     double rescale;             // tau_f / (GR tau_c)
 };
 
@@ -230,7 +230,10 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
     __shared__ P fo[FY][FX];          // output tile of one direction
     const int itm = batch_find(BT.first, BT.n, blockIdx.x);
     const IfaceItem &B = BT.it[itm];
-    const int lb = blockIdx.x - BT.first[itm];
+    const int lbg = blockIdx.x - BT.first[itm];
+    const int G = B.groups, lb = lbg / G, grp = lbg % G;
+    // this workgroup's directions [q0, q1); rho and u (values 0..3) are formed by every group
+    const int q0 = (Q * grp) / G, q1 = (Q * (grp + 1)) / G, nv = 4 + q1 - q0;
     const int Xc = B.Xc, Yc = B.Yc;
     const P rescale = (P)B.rescale;
     const P *__restrict__ Ln = static_cast<const P *>(B.ln);
@@ -256,7 +259,8 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
     for (int k = 0; k < NH; k++)
         if (hok[k]) { a[k][0] = Ln[hoff[k]]; a[k][1] = Lp[hoff[k]]; a[k][2] = Lpp[hoff[k]]; }
     P dens[NP], u0[NP], u1[NP], u2[NP], vv[NP];
-    for (int v = 0; v < NV; v++) {
+    for (int iv = 0; iv < nv; iv++) {
+        const int v = iv < 4 ? iv : 4 + q0 + (iv - 4);
         // 1. Hermite value on the halo tile (tinterpolate_hrmt.c:28-34)
 #pragma unroll
         for (int k = 0; k < NH; k++) {
@@ -269,8 +273,8 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
             const P c4 = ((P)3. * ln - (P)4. * lp + lpp) * (P)0.5;
             s0[p / HX][p % HX] = H.h00 * c1 + H.h10 * c2 + H.h01 * c3 + H.h11 * c4;
         }
-        if (v + 1 < NV) {  // prefetch the next value
-            const size_t vo = (size_t)(v + 1) * n;
+        if (iv + 1 < nv) {  // prefetch the next value
+            const size_t vo = (size_t)(iv + 1 < 4 ? iv + 1 : 4 + q0 + (iv + 1 - 4)) * n;
 #pragma unroll
             for (int k = 0; k < NH; k++)
                 if (hok[k]) { a[k][0] = Ln[vo + hoff[k]]; a[k][1] = Lp[vo + hoff[k]]; a[k][2] = Lpp[vo + hoff[k]]; }
@@ -346,14 +350,18 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
     }
 }
 
-constexpr int IF_TX = 32, IF_TY = 8;
+// coarse tile of one workgroup, and workgroups per tile (the 19 directions can be
+// shared out over several workgroups, each forming rho and u itself; measured on
+// the Re_tau=180 bands, one group is fastest: the fill is bound by the per-value
+// work, which the split multiplies by (4 + 19/G) G / 23)
+constexpr int IF_TX = 32, IF_TY = 8, IF_GROUPS = 1;
 
 template <class S, class P>
 static hipError_t iface_items(hipStream_t s, int GR, const IfaceItem *items, int n, HermiteW<P> H) {
     for (int i0 = 0; i0 < n; i0 += BATCH_MAX) {
         Batch<IfaceItem> B;
         const int blocks = fill_batch(B, items + i0, std::min(BATCH_MAX, n - i0), [](const IfaceItem &a) {
-            return ((a.Xc + IF_TX - 1) / IF_TX) * ((a.Yc + IF_TY - 1) / IF_TY);
+            return ((a.Xc + IF_TX - 1) / IF_TX) * ((a.Yc + IF_TY - 1) / IF_TY) * a.groups;
         });
         const dim3 grd(blocks), blk(IF_TX * IF_TY);
         switch (GR) {
@@ -374,8 +382,8 @@ static void iface_pair(const dnsg_ctx *h, IfaceItem *out) {
     // tauc/((double)GR*tauf) (ctfdtransfer_opt.c:93)
     const double rescale = h->d.omega_c / ((double)h->d.GR * h->d.omega_f);
     const int zds = h->upper.nz;
-    out[0] = {h->lev_lo[0], h->lev_lo[1], h->lev_lo[2], h->lower.n(), zds - 1, h->d.Xc, h->d.Yc, rescale};
-    out[1] = {h->lev_up[0], h->lev_up[1], h->lev_up[2], h->upper.n(), 0, h->d.Xc, h->d.Yc, rescale};
+    out[0] = {h->lev_lo[0], h->lev_lo[1], h->lev_lo[2], h->lower.n(), zds - 1, h->d.Xc, h->d.Yc, IF_GROUPS, rescale};
+    out[1] = {h->lev_up[0], h->lev_up[1], h->lev_up[2], h->upper.n(), 0, h->d.Xc, h->d.Yc, IF_GROUPS, rescale};
 }
 
 hipError_t launch_interface_many(hipStream_t s, dnsg_ctx *const *hs, int n, int iter) {

@@ -10,7 +10,10 @@
 //
 // Convergence: every check_every steps the bulk velocity U = <rho u_x>/<rho>
 // (encalc2.c:109-131 weights) of every resident solve is copied back in one
-// transfer; a solve retires when |U(t) - U(t - window)| <= tol |U(t)|.
+// transfer; a solve retires when |U(t) - U(t - window)| <= tol |U(t)|.  FP32
+// storage leaves U wandering at about 1e-5 relative once the flow is steady, so
+// a shifted-tier solve also retires (status 2) when U has stayed within
+// floor_tol |U| of its values 1..5 windows back and has moved both ways.
 // Slip length: with walls halfway between nodes the channel height is
 // H = (NZs - 2)/GR coarse cells; a no-slip channel under body force G carries
 // U0 = G H^2 / (12 nu_c); equal slip b on both walls raises it to
@@ -141,8 +144,9 @@ int admit(dnsg_queue *q) {
 }
 
 // one coarse step of every resident solve (the kernel sequence of the single-
-// solve step in constant-force mode, without the wall-stress sums)
-int tick(dnsg_queue *q) {
+// solve step in constant-force mode, without the wall-stress sums); the bulk
+// sums are taken only when want_flux (the step before a check)
+int tick(dnsg_queue *q, bool want_flux) {
     for (int prec = 0; prec < 3; prec++) {
         std::vector<dnsg_ctx *> g;
         std::vector<int> slots;
@@ -155,28 +159,34 @@ int tick(dnsg_queue *q) {
             }
         if (g.empty()) continue;
         const int n = (int)g.size();
-        std::vector<LatArgs> L(2 * (size_t)n);
-        for (int i = 0; i < n; i++) {
-            L[i] = coarse_args(g[i]);
-            L[i].red = red_coarse(g[i]);
-        }
-        QCHK(q, launch_collide_stream(q->stream, L.data(), n, prec));
-        for (dnsg_ctx *h : g) rotate_levels(h);
-        QCHK(q, launch_sample_many(q->stream, g.data(), n));
+        std::vector<LatArgs> L(3 * (size_t)n);
         for (int iter = 1; iter <= maxGR; iter++) {
             std::vector<dnsg_ctx *> sub;
             for (dnsg_ctx *h : g)
                 if (h->d.GR >= iter) sub.push_back(h);
             const int m = (int)sub.size();
-            for (int i = 0; i < m; i++) {
-                band_args(sub[i], &L[2 * (size_t)i]);
-                if (iter == 1) {
-                    L[2 * i].red = red_lower(sub[i]);
-                    L[2 * i + 1].red = red_upper(sub[i]);
+            int nl = 0;
+            if (iter == 1) {  // coarse grids and first band substep in one launch set
+                for (int i = 0; i < n; i++) {
+                    L[nl] = coarse_args(g[i]);
+                    if (want_flux) L[nl].red = red_coarse(g[i]);
+                    nl++;
                 }
             }
-            QCHK(q, launch_collide_stream(q->stream, L.data(), 2 * m, prec));
-            if (iter == 1) QCHK(q, launch_flux_many(q->stream, g.data(), n, q->d_u, slots.data()));
+            for (int i = 0; i < m; i++) {
+                band_args(sub[i], &L[nl]);
+                if (iter == 1 && want_flux) {
+                    L[nl].red = red_lower(sub[i]);
+                    L[nl + 1].red = red_upper(sub[i]);
+                }
+                nl += 2;
+            }
+            QCHK(q, launch_collide_stream(q->stream, L.data(), nl, prec));
+            if (iter == 1) {
+                for (dnsg_ctx *h : g) rotate_levels(h);
+                QCHK(q, launch_sample_many(q->stream, g.data(), n));
+                if (want_flux) QCHK(q, launch_flux_many(q->stream, g.data(), n, q->d_u, slots.data()));
+            }
             for (int GR = 1; GR <= 4; GR++) {
                 std::vector<dnsg_ctx *> same;
                 for (dnsg_ctx *h : sub)
@@ -198,6 +208,22 @@ int tick(dnsg_queue *q) {
     return DNSG_OK;
 }
 
+// Steady at the storage precision: U has stayed within bound of its values 1..5
+// windows back, and the 5 window-to-window changes do not all have one sign (a
+// decaying transient moves one way; rounding noise does not).
+bool at_floor(const std::vector<double> &h, size_t W, double bound) {
+    const size_t e = h.size() - 1;
+    const double U = h[e];
+    int up = 0, down = 0;
+    for (size_t j = 1; j <= 5; j++) {
+        if (fabs(U - h[e - j * W]) > bound) return false;
+        const double d = h[e - (j - 1) * W] - h[e - j * W];
+        up += d > 0.;
+        down += d < 0.;
+    }
+    return up > 0 && down > 0;
+}
+
 // read every resident solve's bulk velocity and retire the steady ones
 int check(dnsg_queue *q) {
     const size_t na = q->active.size();
@@ -210,10 +236,14 @@ int check(dnsg_queue *q) {
         const double U = q->h_u[2 * s->slot] / q->h_u[2 * s->slot + 1];
         s->hist.push_back(U);
         const long steps = (long)s->hist.size() * q->o.check_every;
+        const size_t nh = s->hist.size();
         if (!isfinite(U)) {
             retire(q, i, DNSG_ERR_NAN);
-        } else if (s->hist.size() > W && fabs(U - s->hist[s->hist.size() - 1 - W]) <= q->o.tol * fabs(U)) {
+        } else if (nh > W && fabs(U - s->hist[nh - 1 - W]) <= q->o.tol * fabs(U)) {
             retire(q, i, 0);
+        } else if (s->p.precision != DNSG_PREC_FP64 && q->o.floor_tol > 0. && nh > 5 * W &&
+                   at_floor(s->hist, W, q->o.floor_tol * fabs(U))) {
+            retire(q, i, 2);
         } else if (steps >= q->o.max_steps) {
             retire(q, i, 1);
         }
@@ -233,6 +263,7 @@ int dnsg_queue_opts_default(dnsg_queue_opts *o) {
     o->window = 1000;
     o->tol = 1e-9;
     o->max_steps = 200000;
+    o->floor_tol = 3e-5;
     return DNSG_OK;
 }
 
@@ -245,7 +276,7 @@ int dnsg_queue_create(const dnsg_queue_opts *o, dnsg_queue_t **out) {
     q->o = *o;
     q->t0 = std::chrono::steady_clock::now();
     if (o->max_active < 1 || o->max_active > DNSG_QUEUE_MAX_ACTIVE || o->check_every < 1 || o->window < o->check_every ||
-        o->window % o->check_every || !(o->tol >= 0.) || o->max_steps < o->check_every) {
+        o->window % o->check_every || !(o->tol >= 0.) || !(o->floor_tol >= 0.) || o->max_steps < o->check_every) {
         q->err = "invalid queue options";
         return DNSG_ERR_ARG;
     }
@@ -317,7 +348,7 @@ int dnsg_queue_advance(dnsg_queue_t *q, int steps, dnsg_slip_result *done, int c
     int rc = admit(q);
     const int ce = q->o.check_every;
     for (int s = 0; !rc && s < steps && !q->active.empty(); s += ce) {
-        for (int k = 0; k < ce && !rc; k++) rc = tick(q);
+        for (int k = 0; k < ce && !rc; k++) rc = tick(q, k == ce - 1);
         if (!rc) rc = check(q);
         if (!rc) rc = admit(q);
     }

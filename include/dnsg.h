@@ -271,11 +271,16 @@ typedef struct dnsg_queue_opts {
     int window;         /* convergence window in coarse steps */
     double tol;         /* converged when |U(t) - U(t - window)| <= tol |U(t)| */
     long max_steps;     /* a solve still moving after this many steps returns with status 1 */
+    double floor_tol;   /* shifted tiers: steady at the storage precision when, after at least
+                           5 windows, |U(t) - U(t - j window)| <= floor_tol |U(t)| for j = 1..5
+                           and the 5 window-to-window changes have both signs (FP32 rounding
+                           leaves U wandering near 1e-5 relative); 0 disables */
 } dnsg_queue_opts;
 
 typedef struct dnsg_slip_result {
     uint64_t id;
-    int status;         /* 0 converged, 1 step cap reached, < 0 a dnsg error code */
+    int status;         /* 0 converged to tol, 2 steady within floor_tol (shifted tiers),
+                           1 step cap reached, < 0 a dnsg error code */
     int precision;
     long steps;         /* coarse steps taken */
     double ubulk;       /* <rho u_x> / <rho> over the channel (encalc2.c:109 weights) */
@@ -284,7 +289,7 @@ typedef struct dnsg_slip_result {
     double t_arrival, t_admit, t_done; /* seconds on the queue clock */
 } dnsg_slip_result;
 
-/* window 1000, check_every 100, tol 1e-9, max_steps 200000, max_active 64, budget 0 */
+/* window 1000, check_every 100, tol 1e-9, floor_tol 3e-5, max_steps 200000, max_active 64, budget 0 */
 int dnsg_queue_opts_default(dnsg_queue_opts *o);
 int dnsg_queue_create(const dnsg_queue_opts *o, dnsg_queue_t **out);
 int dnsg_queue_destroy(dnsg_queue_t *q);

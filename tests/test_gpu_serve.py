@@ -57,8 +57,8 @@ def test_alone_equals_mix():
         q.submit(p, 20 + i)
     mix = _drain(q, 5)
     q.close()
-    assert alone["status"] == 0, alone
-    assert mix[0]["status"] == 0
+    assert alone["status"] in (0, 2), alone  # FP32: converged or steady at its noise floor
+    assert mix[0]["status"] == alone["status"]
     assert mix[0]["steps"] == alone["steps"]
     assert mix[0]["ubulk"] == alone["ubulk"]
     assert mix[0]["slip_length"] == alone["slip_length"]
@@ -124,7 +124,7 @@ def test_admission_respects_budget():
         assert info["resident_bytes"] <= 2 * per + 1
     q.close()
     assert peak <= 2
-    assert all(r["status"] == 0 for r in out.values())
+    assert all(r["status"] in (0, 2) for r in out.values())
     assert all(r["t_admit"] >= r["t_arrival"] and r["t_done"] > r["t_admit"] for r in out.values())
     with pytest.raises(dnsg.DnsgError):
         q2 = dnsg.Queue(hbm_budget=per // 2)
@@ -138,5 +138,5 @@ def test_fp32_slip_matches_fp64():
     res = _drain(q, 2)
     q.close()
     b64, b32 = res[0]["slip_length"], res[1]["slip_length"]
-    assert res[0]["status"] == 0 and res[1]["status"] == 0, res
+    assert res[0]["status"] == 0 and res[1]["status"] in (0, 2), res
     assert abs(b32 - b64) <= 1e-3 * max(1.0, abs(b64)), (b32, b64)
