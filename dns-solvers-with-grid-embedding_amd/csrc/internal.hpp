@@ -67,8 +67,18 @@ __device__ __forceinline__ int batch_find(const int *first, int n, int b) {
 // is set: sum rho, sum rho u_x, sum u_x (u_x = rho u_x / rho)
 constexpr int RED_W = 3;
 // threads (= nodes) per collide-stream workgroup and chunks per lattice row
-constexpr int CS_THREADS = 256;
+#ifndef DNSG_CS_THREADS
+#define DNSG_CS_THREADS 256
+#endif
+constexpr int CS_THREADS = DNSG_CS_THREADS;
 __host__ __device__ __forceinline__ int cs_chunks(int nx) { return (nx + CS_THREADS - 1) / CS_THREADS; }
+// lattices narrower than a workgroup: each workgroup takes this many whole rows
+__host__ __device__ __forceinline__ int cs_rows_per_wg(int nx) { return nx >= CS_THREADS ? 1 : CS_THREADS / nx; }
+// collide-stream workgroups of one lattice with `rows` rows of nx nodes
+__host__ __device__ __forceinline__ int cs_blocks(int nx, int rows) {
+    const int rpw = cs_rows_per_wg(nx);
+    return rpw == 1 ? rows * cs_chunks(nx) : (rows + rpw - 1) / rpw;
+}
 
 struct Grid {
     void *buf[2] = {nullptr, nullptr};  // A/B lattices

@@ -3,7 +3,8 @@
 //
 // One workgroup owns a chunk of CS_THREADS nodes of one (y, z) row: its reads
 // are 19 contiguous vectors of the row block, its writes the same vectors of
-// the 9 neighbouring rows shifted by e_x.
+// the 9 neighbouring rows shifted by e_x.  A lattice narrower than CS_THREADS
+// packs CS_THREADS / nx whole rows into each workgroup instead.
 //
 // Boundary rules at the stored z ends (xfcomputations.c:215-442):
 //   BOUND_DROP  populations leaving the lattice are not written (the bounded
@@ -45,89 +46,247 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     return (b < (per << 3)) ? (b & 7) * per + (b >> 3) : b;
 }
 
+// Per storage type: use the aligned-store form (bit 0 FP64, 1 FP32, 2 FP16) and,
+// in it, non-temporal stores.  Measured on the Re_tau=180 lattices: the aligned
+// form with non-temporal stores is fastest in FP64; FP32 and FP16 are fastest in
+// the scatter form.
+#ifndef DNSG_CS_ALIGNED_MASK
+#define DNSG_CS_ALIGNED_MASK 1
+#endif
+#ifndef DNSG_CS_NT_MASK
+#define DNSG_CS_NT_MASK 1
+#endif
+template <class S>
+constexpr int prec_bit() { return sizeof(S) == 8 ? 1 : (sizeof(S) == 4 ? 2 : 4); }
+template <class S>
+constexpr bool cs_aligned() { return (DNSG_CS_ALIGNED_MASK & prec_bit<S>()) != 0; }
+template <class S>
+constexpr bool cs_nt() { return (DNSG_CS_NT_MASK & prec_bit<S>()) != 0; }
+template <class S, class V>
+__device__ __forceinline__ void store_cs(S *p, V v) {
+    if constexpr (cs_nt<S>()) store_nt(p, v);
+    else store_s(p, v);
+}
+
+// fused-reduction sums of one thread's nodes (pre-collision moments)
+struct RowSums {
+    double rho = 0., mom = 0., u = 0.;
+    template <class S, class T>
+    __device__ __forceinline__ void add(const T *f) {
+        double d, m;
+        if constexpr (sizeof(S) == 8) {
+            m = first_moment<0>(f);
+            d = zeroth_moment(f);
+        } else {
+            m = (double)first_moment<0>(f);
+            d = 1. + (double)zeroth_moment(f);
+        }
+        rho += d;
+        mom += m;
+        u += m / d;
+    }
+};
+
+// Scatter form for the node at x of row (y, z): collide, then write each
+// direction to its target (x + e_x, y + e_y, z + e_z) with the wall rules.
+template <class S, class T>
+__device__ __forceinline__ void cs_scatter(const LatArgs &L, const S *__restrict__ cur, S *__restrict__ nxt, int row,
+                                           int x, T G, T om, bool red, RowSums &rs) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int y = row % ny, z = row / ny;
+    const bool lo_edge = (z == 0), hi_edge = (z == nz - 1);
+    const int ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
+    T f[Q];
+    unroll<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q] = load_s(cur + (size_t)q * nx + x);
+    });
+    if (red) rs.add<S>(f);
+    collide_any(f, G, om);
+    const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
+    unroll<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const int xt = (EX[q] > 0) ? xp : (EX[q] < 0 ? xm : x);
+        const int yt = (EY[q] > 0) ? yp : (EY[q] < 0 ? ym : y);
+        if constexpr (EZ[q] != 0) {
+            const bool at_end = (EZ[q] < 0) ? lo_edge : hi_edge;
+            if (at_end) {
+                const int kind = (EZ[q] < 0) ? L.lo_kind : L.hi_kind;
+                if (kind == BOUND_WALL) {
+                    const int8_t *wc = (EZ[q] < 0) ? L.wlo : L.whi;
+                    const int sld = wc[(size_t)yt * nx + xt];
+                    const int sld1 = wc[(size_t)y * nx + x];
+                    if (sld == -1 && sld1 == -1)
+                        store_s(nxt + lat_at(OPP[q], (size_t)row, x, nx), f[q]);
+                    else
+                        store_s(nxt + lat_at(MIR[q], (size_t)z * ny + yt, xt, nx), f[q]);
+                }
+                return;
+            }
+        }
+        const int zt = z + EZ[q];
+        if ((zt == 0 && L.skip_lo) || (zt == nz - 1 && L.skip_hi)) return;
+        store_s(nxt + lat_at(q, (size_t)zt * ny + yt, xt, nx), f[q]);
+    });
+}
+
+// Aligned form for chunk x0 .. x0 + CS_THREADS - 1 of row (y, z) (nx a multiple of
+// CS_THREADS): the workgroup also collides the two nodes just outside its chunk
+// and passes the e_x != 0 values one lane along (shuffles within a wave, LDS
+// between waves), so that every store of direction q lands on the workgroup's own
+// positions of the target row: whole cache lines, written once.  Each thread
+// stores, at its own x, the value arriving there from x - e_x; the set of writes
+// is exactly the scatter form's.
+template <class S, class T>
+__device__ __forceinline__ void cs_aligned_chunk(const LatArgs &L, const S *__restrict__ cur, S *__restrict__ nxt,
+                                                 int row, int x0, T G, T om, bool red, RowSums &rs) {
+    constexpr int NW = CS_THREADS / 64;
+    __shared__ T hb[2][Q];      // post-collision halo nodes x0 - 1, x0 + CS_THREADS
+    __shared__ T wb[NW][2][Q];  // per wave: last lane's e_x > 0, first lane's e_x < 0 values
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int y = row % ny, z = row / ny;
+    const bool lo_edge = (z == 0), hi_edge = (z == nz - 1);
+    const int ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, x = x0 + t;
+    if (t == 0 || t == CS_THREADS - 1) {
+        const int side = t == 0 ? 0 : 1;
+        int xh = side == 0 ? x0 - 1 : x0 + CS_THREADS;
+        xh = xh < 0 ? xh + nx : (xh >= nx ? xh - nx : xh);
+        T g[Q];
+        unroll<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            g[q] = load_s(cur + (size_t)q * nx + xh);
+        });
+        collide_any(g, G, om);
+        unroll<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (EX[q] != 0) hb[side][q] = g[q];
+        });
+    }
+    T f[Q];
+    unroll<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q] = load_s(cur + (size_t)q * nx + x);
+    });
+    if (red) rs.add<S>(f);
+    collide_any(f, G, om);
+    unroll<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (EX[q] > 0) {
+            if (lane == 63) wb[w][0][q] = f[q];
+        } else if constexpr (EX[q] < 0) {
+            if (lane == 0) wb[w][1][q] = f[q];
+        }
+    });
+    __syncthreads();
+    const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
+    unroll<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        T v = f[q];  // the value arriving at x: post-collision f_q of node x - e_x
+        if constexpr (EX[q] > 0) {
+            v = __shfl_up(f[q], 1, 64);
+            if (lane == 0) v = (w == 0) ? hb[0][q] : wb[w - 1][0][q];
+        } else if constexpr (EX[q] < 0) {
+            v = __shfl_down(f[q], 1, 64);
+            if (lane == 63) v = (w == NW - 1) ? hb[1][q] : wb[w + 1][1][q];
+        }
+        const int yt = (EY[q] > 0) ? yp : (EY[q] < 0 ? ym : y);
+        if constexpr (EZ[q] != 0) {
+            const bool at_end = (EZ[q] < 0) ? lo_edge : hi_edge;
+            if (at_end) {
+                const int kind = (EZ[q] < 0) ? L.lo_kind : L.hi_kind;
+                if (kind == BOUND_WALL) {
+                    const int8_t *wc = (EZ[q] < 0) ? L.wlo : L.whi;
+                    // specular reflection of the source x - e_x into (x, yt), unless it bounces back
+                    const int xs = (EX[q] > 0) ? xm : (EX[q] < 0 ? xp : x);
+                    if (!(wc[(size_t)yt * nx + x] == -1 && wc[(size_t)y * nx + xs] == -1))
+                        store_cs(nxt + lat_at(MIR[q], (size_t)z * ny + yt, x, nx), v);
+                    // halfway bounce-back of this node's own value
+                    const int xt = (EX[q] > 0) ? xp : (EX[q] < 0 ? xm : x);
+                    if (wc[(size_t)yt * nx + xt] == -1 && wc[(size_t)y * nx + x] == -1)
+                        store_cs(nxt + lat_at(OPP[q], (size_t)row, x, nx), f[q]);
+                }
+                return;
+            }
+        }
+        const int zt = z + EZ[q];
+        if ((zt == 0 && L.skip_lo) || (zt == nz - 1 && L.skip_hi)) return;
+        store_cs(nxt + lat_at(q, (size_t)zt * ny + yt, x, nx), v);
+    });
+}
+
+__device__ __forceinline__ void write_sums(double *red, size_t e, const RowSums &s) {
+    red[e + 0] = s.rho;
+    red[e + 1] = s.mom;
+    red[e + 2] = s.u;
+}
+
+// workgroup sum of one partial (wave shuffles, then LDS over the waves)
+__device__ __forceinline__ RowSums block_sums(RowSums s) {
+    __shared__ double lds[3][CS_THREADS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    s.rho = wave_sum64(s.rho);
+    s.mom = wave_sum64(s.mom);
+    s.u = wave_sum64(s.u);
+    if (lane == 0) { lds[0][wid] = s.rho; lds[1][wid] = s.mom; lds[2][wid] = s.u; }
+    __syncthreads();
+    RowSums t;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < CS_THREADS / 64; w++) { t.rho += lds[0][w]; t.mom += lds[1][w]; t.u += lds[2][w]; }
+    return t;
+}
+
 template <class S>
 __global__ __launch_bounds__(CS_THREADS) void k_collide_stream(Batch<LatArgs> P) {
+    using T = typename ComputeOf<S>::T;
     const int bg = xcd_remap(blockIdx.x, gridDim.x);
     const int it = batch_find(P.first, P.n, bg);
     const LatArgs &L = P.it[it];
-    const int nx = L.nx, ny = L.ny, nz = L.nz;
-    const int cpr = (nx + CS_THREADS - 1) / CS_THREADS;  // chunks per row
+    const int nx = L.nx, ny = L.ny;
     const int b = bg - P.first[it];
-    const int row = b / cpr, chunk = b % cpr;
-    const double *__restrict__ force = L.force;
-    using T = typename ComputeOf<S>::T;
-    const int y = row % ny, z = row / ny;
-    const S *__restrict__ cur = static_cast<const S *>(L.cur) + (size_t)row * Q * nx;
+    const S *__restrict__ base = static_cast<const S *>(L.cur);
     S *__restrict__ nxt = static_cast<S *>(L.nxt);
-    const T G = (T)(L.gscale * force[0]);
+    const T G = (T)(L.gscale * L.force[0]);
     const T om = (T)L.omega;
-    const bool lo_edge = (z == 0), hi_edge = (z == nz - 1);
-    const int ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
     const bool red = L.red != nullptr;
-    double r_rho = 0., r_mom = 0., r_u = 0.;
-    const int x = chunk * CS_THREADS + threadIdx.x;
-    if (x < nx) {  // a partial last chunk still joins the reduction below
-        T f[Q];
-        unroll<0, Q>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            f[q] = load_s(cur + (size_t)q * nx + x);
-        });
-        if (red) {  // moments of the pre-collision state, for the fused reductions
-            double d = 0., m = 0.;
-            if constexpr (sizeof(S) == 8) {
-                m = first_moment<0>(f);
-                d = zeroth_moment(f);
-            } else {
-                m = (double)first_moment<0>(f);
-                d = 1. + (double)zeroth_moment(f);
-            }
-            r_rho += d;
-            r_mom += m;
-            r_u += m / d;
+    const int rpw = cs_rows_per_wg(nx);
+    RowSums rs;
+    if (rpw == 1) {
+        // one (row, chunk) per workgroup: row, y and z are uniform over the workgroup
+        const int cpr = cs_chunks(nx);
+        const int row = b / cpr, chunk = b % cpr;
+        const S *__restrict__ cur = base + (size_t)row * Q * nx;
+        if (cs_aligned<S>() && nx % CS_THREADS == 0) {
+            cs_aligned_chunk<S, T>(L, cur, nxt, row, chunk * CS_THREADS, G, om, red, rs);
+        } else {
+            const int x = chunk * CS_THREADS + threadIdx.x;
+            if (x < nx) cs_scatter<S, T>(L, cur, nxt, row, x, G, om, red, rs);  // a partial chunk still sums below
         }
-        collide_any(f, G, om);
-        const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
-        unroll<0, Q>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            const int xt = (EX[q] > 0) ? xp : (EX[q] < 0 ? xm : x);
-            const int yt = (EY[q] > 0) ? yp : (EY[q] < 0 ? ym : y);
-            if constexpr (EZ[q] != 0) {
-                const bool at_end = (EZ[q] < 0) ? lo_edge : hi_edge;
-                if (at_end) {
-                    const int kind = (EZ[q] < 0) ? L.lo_kind : L.hi_kind;
-                    if (kind == BOUND_WALL) {
-                        const int8_t *wc = (EZ[q] < 0) ? L.wlo : L.whi;
-                        const int sld = wc[(size_t)yt * nx + xt];
-                        const int sld1 = wc[(size_t)y * nx + x];
-                        if (sld == -1 && sld1 == -1)
-                            store_s(nxt + lat_at(OPP[q], (size_t)row, x, nx), f[q]);
-                        else
-                            store_s(nxt + lat_at(MIR[q], (size_t)z * ny + yt, xt, nx), f[q]);
-                    }
-                    return;
+        if (red) {  // uniform per workgroup
+            const RowSums t = block_sums(rs);
+            if (threadIdx.x == 0) write_sums(L.red, ((size_t)row * cpr + chunk) * RED_W, t);
+        }
+    } else {
+        // narrow lattice: rpw whole rows per workgroup, one partial per row summed in x order
+        const int r = threadIdx.x / nx, x = threadIdx.x - r * nx;
+        const int row = b * rpw + r;
+        if (r < rpw && row < ny * L.nz) cs_scatter<S, T>(L, base + (size_t)row * Q * nx, nxt, row, x, G, om, red, rs);
+        if (red) {
+            __shared__ double lr[3][CS_THREADS];
+            lr[0][threadIdx.x] = rs.rho;
+            lr[1][threadIdx.x] = rs.mom;
+            lr[2][threadIdx.x] = rs.u;
+            __syncthreads();
+            const int rr = b * rpw + (int)threadIdx.x;
+            if ((int)threadIdx.x < rpw && rr < ny * L.nz) {
+                RowSums t;
+                for (int k = threadIdx.x * nx; k < (int)(threadIdx.x + 1) * nx; k++) {
+                    t.rho += lr[0][k];
+                    t.mom += lr[1][k];
+                    t.u += lr[2][k];
                 }
+                write_sums(L.red, (size_t)rr * RED_W, t);
             }
-            const int zt = z + EZ[q];
-            if ((zt == 0 && L.skip_lo) || (zt == nz - 1 && L.skip_hi)) return;
-            store_s(nxt + lat_at(q, (size_t)zt * ny + yt, xt, nx), f[q]);
-        });
-    }
-    if (red) {  // uniform per workgroup
-        __shared__ double lds[3][4];
-        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-        r_rho = wave_sum64(r_rho);
-        r_mom = wave_sum64(r_mom);
-        r_u = wave_sum64(r_u);
-        if (lane == 0) { lds[0][wid] = r_rho; lds[1][wid] = r_mom; lds[2][wid] = r_u; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const int nw = (blockDim.x + 63) >> 6;
-            double a = 0., bb = 0., c = 0.;
-            for (int w = 0; w < nw; w++) { a += lds[0][w]; bb += lds[1][w]; c += lds[2][w]; }
-            const size_t e = ((size_t)row * cpr + chunk) * RED_W;
-            L.red[e + 0] = a;
-            L.red[e + 1] = bb;
-            L.red[e + 2] = c;
         }
     }
 }
@@ -140,7 +299,7 @@ hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, i
         for (int i = 0; i < BATCH_MAX; i++) {
             P.it[i] = lats[i0 + (i < P.n ? i : 0)];
             P.first[i] = blocks;
-            if (i < P.n) blocks += P.it[i].ny * P.it[i].nz * cs_chunks(P.it[i].nx);
+            if (i < P.n) blocks += cs_blocks(P.it[i].nx, P.it[i].ny * P.it[i].nz);
         }
         P.first[BATCH_MAX] = blocks;
         if (blocks == 0) continue;

@@ -161,5 +161,12 @@ __device__ __forceinline__ double load_s(const double *p) { return *p; }
 __device__ __forceinline__ void store_s(float *p, float v) { *p = v; }
 __device__ __forceinline__ void store_s(__half *p, float v) { *p = __float2half(v); }
 __device__ __forceinline__ void store_s(double *p, double v) { *p = v; }
+// streaming stores that bypass the caches' normal allocation (written once, read next step)
+__device__ __forceinline__ void store_nt(float *p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void store_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void store_nt(__half *p, float v) {
+    const __half h = __float2half(v);
+    __builtin_nontemporal_store(*reinterpret_cast<const unsigned short *>(&h), reinterpret_cast<unsigned short *>(p));
+}
 
 }  // namespace dnsg

@@ -29,6 +29,11 @@ CASES = {
     "gr3": dict(MICRO, gratio=3, zds=5, pattern="rslip", gyplussize=6, wyplussize=6),
     "ragged": dict(xdv=13, ydv=7, zdv=9, gratio=2, zds=5, Rebs=20.0, ub=0.03, CFL=5.0,
                    pattern="pslip", gxplussize=2, wxplussize=24, gyplussize=5, wyplussize=2),
+    # rows of 256 (coarse) and 512 (bands) nodes: the collide-stream's aligned-store form
+    "wide_pslip": dict(xdv=256, ydv=8, zdv=10, gratio=2, zds=4, Rebs=20.0, ub=0.0333333333333, CFL=5.0,
+                       pattern="pslip", gxplussize=8, wxplussize=8, gyplussize=4, wyplussize=4),
+    "wide_sslip": dict(xdv=256, ydv=6, zdv=9, gratio=2, zds=4, Rebs=20.0, ub=0.0333333333333, CFL=2.0,
+                       pattern="sslip", gxplussize=3, wxplussize=5),
 }
 INIT = dict(init_kind="laminar_noise", seed=42, u_max=0.05, noise=0.05)
 GRIDS = ("coarse", "lower", "upper")
