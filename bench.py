@@ -250,6 +250,7 @@ def main():
     ap.add_argument("--queue-rate", type=float, default=1.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--queue-check", type=int, default=8, help="requests re-solved in FP64 for the slip check")
     ap.add_argument("--no-queue", action="store_true")
+    ap.add_argument("--no-precision-report", action="store_true")
     ap.add_argument("--pmc-run", action="store_true",
                     help="minimal run for hardware-counter passes: 3 FP64 steps of the workload, then one "
                          "1 GiB copy-probe launch whose known bytes calibrate FETCH_SIZE / WRITE_SIZE")
@@ -285,8 +286,10 @@ def main():
                 frac=(bl["gbs"] / HBM_PEAK_GBS) if bl["gbs"] else None, traffic=traffic,
                 kernel="k_collide_stream<double> (both wall bands, one launch per substep)",
                 algorithmic_bytes_per_launch=bl["bytes"], launch_ms=bl["ms"])
-    print("bench: precision tier report (64x32x32 channel, 500 steps)", file=sys.stderr, flush=True)
-    prec_report = precision_tier(mod)
+    prec_report = None
+    if not args.no_precision_report:
+        print("bench: precision tier report (64x32x32 channel, 500 steps)", file=sys.stderr, flush=True)
+        prec_report = precision_tier(mod)
     queue = None
     if not args.no_queue and args.queue_requests > 0:
         print(f"bench: slip queue ({args.queue_requests} requests at {args.queue_rate}/s)", file=sys.stderr, flush=True)
