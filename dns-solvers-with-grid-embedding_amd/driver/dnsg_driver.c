@@ -8,7 +8,8 @@
  * `#define NAME value` lines are parsed, including the RSLIP/PSLIP/SSLIP and
  * ITER switches) and/or key=value arguments with the dnsg_params field names,
  * which also reach the runtime-only extensions (force_mode, g_const, tau_c,
- * precision, init_kind, seed, u_max, noise, results_dat).
+ * precision, init_kind, seed, u_max, noise, results_dat).  print_params=1 prints
+ * the parsed parameters and the derived sizes and exits without GPU work.
  *
  * Run modes (main:944-1190): -s starts from rest ('-s' of the reference) or,
  * with init_kind=1, from the seeded laminar profile with noise; -p reads the
@@ -38,6 +39,7 @@ static const double E[3][19] = {{0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, 1, -1, -
                                 {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1}};
 
 static int results_dat = 1;
+static int print_params = 0;
 
 static double now(void) {
     struct timespec t;
@@ -62,6 +64,7 @@ static int set_param(dnsg_params *p, const char *key, const char *val) {
     if (!strcmp(key, "ITER")) { p->ITER = strtol(val, NULL, 10); return 0; }
     if (!strcmp(key, "seed")) { p->seed = strtoull(val, NULL, 10); return 0; }
     if (!strcmp(key, "results_dat")) { results_dat = atoi(val); return 0; }
+    if (!strcmp(key, "print_params")) { print_params = atoi(val); return 0; }
     if (!strcmp(key, "XDIM") || !strcmp(key, "YDIM")) return 0; /* one GPU = one rank */
     return -1;
 }
@@ -333,6 +336,16 @@ int main(int argc, char **argv) {
             key[n] = 0;
             if (set_param(&p, key, eq + 1)) { fprintf(stderr, "unknown parameter %s\n", key); return 2; }
         }
+    }
+    if (print_params) { /* the parsed parameters and derived sizes; no GPU work */
+        if ((rc = dnsg_dims_from_params(&p, &d))) { fprintf(stderr, "invalid parameters (%d)\n", rc); return 2; }
+        printf("Rebs=%.17g\nub=%.17g\nxdv=%d\nydv=%d\nzdv=%d\nTSTR=%d\nTEND=%d\nCFL=%.17g\ngratio=%d\nzds=%d\n",
+               p.Rebs, p.ub, p.xdv, p.ydv, p.zdv, p.TSTR, p.TEND, p.CFL, p.gratio, p.zds);
+        printf("gyplussize=%d\nwyplussize=%d\ngxplussize=%d\nwxplussize=%d\npattern=%d\nITER=%ld\n",
+               p.gyplussize, p.wyplussize, p.gxplussize, p.wxplussize, p.pattern, p.ITER);
+        printf("force_mode=%d\nprecision=%d\ninit_kind=%d\nNZs=%d\nNZ=%d\nutime=%ld\nup=%ld\ntau_c=%.17g\ntau_f=%.17g\n",
+               p.force_mode, p.precision, p.init_kind, d.NZs, d.NZ, d.utime, d.up, d.tau_c, d.tau_f);
+        return 0;
     }
     if ((rc = dnsg_create(&p, &h))) die(h, "dnsg_create", rc);
     dnsg_get_dims(h, &d);

@@ -105,6 +105,70 @@ def test_driver_rest_start(tmp_path, oracle_built):
     _compare(odir, gdir, exact_state=False, tol=1e-11)
 
 
+DEFINITIONS = """/* a run configuration in the format of the reference's definitions.h */
+#define Rebs 20.0
+#define ub 0.0333333333333
+#define zdv  12
+#define ydv  8
+#define xdv  16
+#define TSTR 0
+#define TEND 1
+#define CFL  5.0
+#define gratio 2
+#define zds 4
+#define gyplussize 4
+#define wyplussize 4
+#define gxplussize 4
+#define wxplussize 4
+//#define PSLIP
+#define RSLIP
+//#define SSLIP
+#define XDIM 4
+#define YDIM 2
+#define ITER 50
+#define STRIPE_COUNT "64"
+#define Fs(a,i,j,k,xd,yd,zd) (*(a+(i*yd*zd)+(j*zd)+k))
+"""
+
+
+def test_driver_parses_definitions(tmp_path):
+    """`#define` lines, the pattern switches and ITER are read; comments, macros with
+    arguments and the MPI/IO settings are ignored; the derived sizes are the oracle's"""
+    if not os.path.exists(DRIVER):
+        pytest.skip("driver not built")
+    (tmp_path / "definitions.h").write_text(DEFINITIONS)
+    r = subprocess.run([DRIVER, "-s", str(tmp_path / "definitions.h"), "print_params=1"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=30)
+    assert r.returncode == 0, r.stderr
+    got = dict(line.split("=", 1) for line in r.stdout.split())
+    assert got["pattern"] == "1" and got["ITER"] == "50" and got["xdv"] == "16" and got["zds"] == "4"
+    assert float(got["ub"]) == UB and float(got["CFL"]) == 5.0 and got["force_mode"] == "0"
+    from helpers import Oracle, dnsg
+    o = Oracle(dnsg.make_params(**MICRO))
+    assert int(got["NZs"]) == o.dims.NZs and int(got["NZ"]) == o.dims.NZ
+    assert int(got["utime"]) == o.dims.utime and int(got["up"]) == o.dims.up
+    assert float(got["tau_c"]) == o.dims.tau_c and float(got["tau_f"]) == o.dims.tau_f
+    o.close()
+
+
+@pytest.mark.gpu
+def test_driver_runs_from_definitions_file(tmp_path, oracle_built):
+    """`dnsg-run -s definitions.h` (the reference's own invocation) against the oracle"""
+    kw = dict(MICRO, init_kind=0)
+    odir, gdir = tmp_path / "oracle", tmp_path / "driver"
+    odir.mkdir()
+    gdir.mkdir()
+    subprocess.run([ORC_CLI, "init", str(odir)] + _args(kw), check=True)
+    subprocess.run([ORC_CLI, "run", str(odir)] + _args(kw) + ["threads=4"], check=True, capture_output=True)
+    (gdir / "definitions.h").write_text(DEFINITIONS)
+    print("driver: dnsg-run -s definitions.h", flush=True)
+    r = subprocess.run([DRIVER, "-s", "definitions.h", "results_dat=0"], cwd=gdir, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    (gdir / "definitions.h").unlink()
+    _compare(odir, gdir, exact_state=False, tol=1e-11)
+
+
 def test_driver_rejects_unknown_key(tmp_path):
     if not os.path.exists(DRIVER):
         pytest.skip("driver not built")
