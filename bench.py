@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_LU = {"fp64": 304, "fp32": 152, "fp16": 76}
 
 
-def workload_params(mod, prec="fp64"):
+def workload_params(mod, prec="fp64", force_mode="const"):
     GR, zds, zdv = 2, 16, 128
     zdf = zds + 1
     NZs = 2 * zdf + GR * (zdv - 4) + GR - 1
@@ -50,7 +50,7 @@ def workload_params(mod, prec="fp64"):
     G = u_tau * u_tau / h
     ub = 15.6 * u_tau                   # bulk velocity of the laminar start (Dean: U_b ~ 15.6 u_tau)
     return mod.make_params(xdv=512, ydv=256, zdv=zdv, gratio=GR, zds=zds, tau_c=tau_c, ub=ub,
-                           pattern="rslip", gyplussize=32, wyplussize=32, force_mode="const",
+                           pattern="rslip", gyplussize=32, wyplussize=32, force_mode=force_mode,
                            g_const=G, precision=prec, init_kind="laminar_noise", seed=2024,
                            u_max=1.5 * ub, noise=0.10, CFL=0.1)
 
@@ -59,8 +59,8 @@ def lu_per_step(d):
     return d.Xc * d.Yc * d.zdc + d.GR * 2 * d.Xf * d.Yf * (d.zdf - 1)
 
 
-def run_gpu(mod, prec, steps, warmup):
-    p = workload_params(mod, prec)
+def run_gpu(mod, prec, steps, warmup, force_mode="const"):
+    p = workload_params(mod, prec, force_mode)
     s = mod.Solver(p)
     s.init_state()
     if warmup:
@@ -276,6 +276,13 @@ def main():
         res[prec] = run_gpu(mod, prec, args.steps, args.warmup)
         print(f"bench: {prec} {res[prec]['glups']:.3f} GLUPS, {res[prec]['ms_per_step']:.3f} ms/step, "
               f"kernels {res[prec]['kernels_ms']}", file=sys.stderr, flush=True)
+    flux = None
+    if "fp64" in tiers:  # the reference's own driving: G = ub - <rho u> every step
+        print(f"bench: fp64 flux controller {args.steps} steps", file=sys.stderr, flush=True)
+        r = run_gpu(mod, "fp64", args.steps, args.warmup, force_mode="flux")
+        flux = {"glups": round(r["glups"], 4), "ms_per_step": round(r["ms_per_step"], 4),
+                "kernels_ms": {a: round(b, 4) for a, b in r["kernels_ms"].items()}}
+        print(f"bench: fp64 flux {r['glups']:.3f} GLUPS", file=sys.stderr, flush=True)
     probe = {k: round(mod.hbm_probe(4 << 30, i, 5), 1) for i, k in enumerate(("copy", "read", "write"))}
     print(f"bench: HBM probe {probe} GB/s", file=sys.stderr, flush=True)
     main_t = res["fp64"] if "fp64" in res else res[tiers[0]]
@@ -319,6 +326,7 @@ def main():
         "hbm_roofline_frac_step": round(main_t["glups"] * BYTES_PER_LU[main_prec] / HBM_PEAK_GBS, 4),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "fp64_flux_controller": flux,
         "slip_queue": queue,
         "precision_tier": prec_report,
         "hbm_probe_gbs": probe,

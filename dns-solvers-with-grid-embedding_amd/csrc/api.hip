@@ -151,6 +151,8 @@ LatArgs coarse_args(dnsg_t *h) {
     L.skip_hi = 1;
     L.red = nullptr;
     L.force = h->d_force;
+    L.dsum = nullptr;
+    set_dest_weights(h, 0, &L);
     return L;
 }
 
@@ -173,6 +175,8 @@ void band_args(dnsg_t *h, LatArgs *L) {
         L[b].skip_hi = b == 0 ? 1 : 0;
         L[b].red = nullptr;
         L[b].force = h->d_force;
+        L[b].dsum = nullptr;
+        set_dest_weights(h, 1 + b, &L[b]);
     }
 }
 
@@ -232,38 +236,59 @@ struct SecTimer {
 // (xcomputations and the first xfcomputations of the reference's loop).
 int do_step(dnsg_t *h, double *rec, SecTimer *T = nullptr) {
     const bool flux = h->p.force_mode == DNSG_FORCE_FLUX;
+    // flux mode: G for this step comes from the previous step's writes when that
+    // step ran here (flux_fused), else from a pass over the lattices (first step,
+    // or after the state was changed from outside)
+    const bool from_writes = flux && h->flux_fused;
+    const int GR = h->d.GR;
     if (T) T->mark(h->stream, -1);
-    if (flux) {
+    if (flux && !from_writes) {
         HIPCHK(h, launch_encalc2(h->stream, h, rec));
         if (T) T->mark(h->stream, SEC_REDUCE);
     }
-    for (int iter = 1; iter <= h->d.GR; iter++) {
+    double *dl = h->dsum + 2 * (size_t)h->d.Yc * h->d.zdc * cs_chunks(h->d.Xc);
+    double *du = dl + 2 * (size_t)h->d.Yf * (h->d.zdf - 1) * cs_chunks(h->d.Xf);
+    for (int iter = 1; iter <= GR; iter++) {
         LatArgs L[3];
+        LatArgs *B = L;  // the band pair of this substep
+        int nl = 2;
         if (iter == 1) {
             L[0] = coarse_args(h);
-            band_args(h, &L[1]);
+            B = L + 1;
+            nl = 3;
             if (!flux) L[0].red = red_coarse(h);
-            L[1].red = red_lower(h);
-            L[2].red = red_upper(h);
-            HIPCHK(h, launch_collide_stream(h->stream, L, 3, h->p.precision));
-            if (T) T->mark(h->stream, SEC_COARSE);
+            if (flux) L[0].dsum = h->dsum;
+        }
+        band_args(h, B);
+        if (iter == 1) {
+            B[0].red = red_lower(h);
+            B[1].red = red_upper(h);
+        }
+        if (flux && iter == GR) {  // the last substep writes the bands' next state
+            B[0].dsum = dl;
+            B[1].dsum = du;
+        }
+        HIPCHK(h, launch_collide_stream(h->stream, L, nl, h->p.precision));
+        if (T) T->mark(h->stream, iter == 1 ? SEC_COARSE : SEC_BAND);
+        if (iter == 1) {
             rotate_levels(h);
             HIPCHK(h, launch_sample(h->stream, h, h->coarse.n(), h->lev_lo[0], h->lev_up[0]));
             if (T) T->mark(h->stream, SEC_SAMPLE);
             if (!flux) HIPCHK(h, launch_encalc2_fused(h->stream, h, rec));
-            HIPCHK(h, launch_wshear_fused(h->stream, h, rec));
+            HIPCHK(h, launch_wshear_fused(h->stream, h, rec, from_writes ? h->enc_next : nullptr));
             if (T) T->mark(h->stream, SEC_REDUCE);
-        } else {
-            band_args(h, L);
-            HIPCHK(h, launch_collide_stream(h->stream, L, 2, h->p.precision));
-            if (T) T->mark(h->stream, SEC_BAND);
         }
         HIPCHK(h, launch_interface(h->stream, h, iter));
-        if (iter == h->d.GR) HIPCHK(h, launch_ftcd(h->stream, h));
+        if (iter == GR) HIPCHK(h, launch_ftcd(h->stream, h, flux ? h->fsum : nullptr));
         if (T) T->mark(h->stream, SEC_IFACE);
         h->lower.swap();
         h->upper.swap();
     }
+    if (flux) {
+        HIPCHK(h, launch_flux_next(h->stream, h));
+        if (T) T->mark(h->stream, SEC_REDUCE);
+    }
+    h->flux_fused = flux;
     h->coarse.swap();
     h->ss++;
     return DNSG_OK;
@@ -436,7 +461,8 @@ int dnsg_destroy(dnsg_t *h) {
         if (h->lev_lo[l]) (void)hipFree(h->lev_lo[l]);
         if (h->lev_up[l]) (void)hipFree(h->lev_up[l]);
     }
-    void *ptrs[] = {h->wall_lo, h->wall_up, h->part, h->red, h->seg, h->d_force, h->d_diag, h->d_red, h->d_acc};
+    void *ptrs[] = {h->wall_lo, h->wall_up, h->part, h->red, h->seg, h->dsum, h->fsum, h->enc_next,
+                    h->d_force, h->d_diag, h->d_red, h->d_acc};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (h->h_pinned) (void)hipHostFree(h->h_pinned);
@@ -503,6 +529,9 @@ int dnsg::create_ctx(const dnsg_params *p, hipStream_t ext, dnsg_t **out) {
         HIPCHK(h, hipMalloc(&h->red, nred * 8));
         HIPCHK(h, hipMemsetAsync(h->red, 0, nred * 8, h->stream));
         HIPCHK(h, hipMalloc(&h->seg, enc_seg_doubles(h) * 8));
+        HIPCHK(h, hipMalloc(&h->dsum, dsum_doubles(h) * 8));
+        HIPCHK(h, hipMalloc(&h->fsum, 2 * (size_t)ftcd_blocks(h) * 8));
+        HIPCHK(h, hipMalloc(&h->enc_next, 8 * 8));
     }
     HIPCHK(h, hipMalloc(&h->d_force, 4 * 8));
     HIPCHK(h, hipMemsetAsync(h->d_force, 0, 4 * 8, h->stream));
@@ -534,6 +563,7 @@ static int set_force(dnsg_t *h, double g) {
 
 int dnsg_init_state(dnsg_t *h) {
     if (!h || !h->stream) return DNSG_ERR_ARG;
+    h->flux_fused = false;
     HIPCHK(h, launch_init_lattice(h->stream, h, 0, h->coarse.c()));
     HIPCHK(h, launch_init_lattice(h->stream, h, 1, h->lower.c()));
     HIPCHK(h, launch_init_lattice(h->stream, h, 2, h->upper.c()));
@@ -577,6 +607,7 @@ int dnsg_init_state(dnsg_t *h) {
 
 int dnsg_upload_state(dnsg_t *h, const double *coarse, const double *lower, const double *upper, const double *lin3,
                       const double *uin3, double force) {
+    if (h) h->flux_fused = false;
     if (!h || !h->stream) return DNSG_ERR_ARG;
     int rc;
     const int zdf = h->d.zdf;
@@ -747,7 +778,10 @@ static int scalar_call(dnsg_t *h, long ss, dnsg_diag *d) {
 
 int dnsg_encalc2(dnsg_t *h, long ss, dnsg_diag *d) {
     if (!h || !h->stream) return DNSG_ERR_ARG;
-    HIPCHK(h, launch_encalc2(h->stream, h, h->d_diag));
+    if (h->flux_fused)  // the previous dnsg_step already formed this state's controller output
+        HIPCHK(h, hipMemcpyAsync(h->d_diag + 1, h->enc_next, 4 * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    else
+        HIPCHK(h, launch_encalc2(h->stream, h, h->d_diag));
     scalar_call(h, ss, d);
     const double *rec = h->h_pinned;
     h->force = rec[1];
@@ -768,23 +802,37 @@ int dnsg_wshearstr(dnsg_t *h, long ss, dnsg_diag *d) {
     return DNSG_OK;
 }
 
-int dnsg_xcomputations(dnsg_t *h) { return h ? do_xcomputations(h) : DNSG_ERR_ARG; }
-int dnsg_xfcomputations(dnsg_t *h) { return h ? do_xfcomputations(h) : DNSG_ERR_ARG; }
+// The granular calls change the state outside dnsg_step: the next step's flux
+// controller then sums the lattices again (flux_fused = false).
+int dnsg_xcomputations(dnsg_t *h) {
+    if (!h) return DNSG_ERR_ARG;
+    h->flux_fused = false;
+    return do_xcomputations(h);
+}
+
+int dnsg_xfcomputations(dnsg_t *h) {
+    if (!h) return DNSG_ERR_ARG;
+    h->flux_fused = false;
+    return do_xfcomputations(h);
+}
 
 int dnsg_ctfdtransfer(dnsg_t *h, int iter) {
     if (!h || iter < 1 || iter > h->d.GR) return DNSG_ERR_ARG;
+    h->flux_fused = false;
     HIPCHK(h, launch_interface(h->stream, h, iter));
     return DNSG_OK;
 }
 
 int dnsg_ftcdtransfer(dnsg_t *h) {
     if (!h) return DNSG_ERR_ARG;
+    h->flux_fused = false;
     HIPCHK(h, launch_ftcd(h->stream, h));
     return DNSG_OK;
 }
 
 int dnsg_exchange(dnsg_t *h) {
     if (!h) return DNSG_ERR_ARG;
+    h->flux_fused = false;
     h->coarse.swap();
     h->ss++;
     return DNSG_OK;
@@ -792,6 +840,7 @@ int dnsg_exchange(dnsg_t *h) {
 
 int dnsg_fexchange(dnsg_t *h) {
     if (!h) return DNSG_ERR_ARG;
+    h->flux_fused = false;
     h->lower.swap();
     h->upper.swap();
     return DNSG_OK;

@@ -44,6 +44,12 @@ struct LatArgs {
     int skip_lo, skip_hi;  // do not write into layer 0 / nz-1 (rewritten by the interface)
     double *red;           // if set: per-row sums of the pre-collision state, RED_W per row
     const double *force;   // V.force[0] of the solve this lattice belongs to
+    // if set: per-(row, chunk) sums over the values this launch writes, each weighted
+    // by the encalc2 weight of its destination layer: [sum w e_x f, sum w f].  Layers
+    // klo..khi carry kscale (khalf: half of it), all others 0.
+    double *dsum;
+    int klo, khi, khalf;
+    double kscale;
 };
 
 // Several independent work items in one launch: item i owns the workgroups
@@ -119,6 +125,13 @@ struct dnsg_ctx {
     double *part = nullptr;     // row partials of the explicit reduction passes
     double *red = nullptr;      // sums written by the collide-stream kernels (RED_W per row chunk)
     double *seg = nullptr;      // first-stage segment sums of the flux reduction
+    // flux controller from the writes of the previous step: destination-weighted
+    // sums of the collide-stream launches [coarse | lower | upper] (2 per row chunk)
+    // and of the injection (2 per workgroup); enc_next holds the next step's
+    // G, <rho u>, <rho>, flowrate; flux_fused says d_force and enc_next belong to the
+    // current state
+    double *dsum = nullptr, *fsum = nullptr, *enc_next = nullptr;
+    bool flux_fused = false;
     size_t part_cap = 0;        // doubles
     double *d_force = nullptr;  // device copy of V.force[0]
     double *d_diag = nullptr;   // DIAG_WORDS per step
@@ -162,18 +175,25 @@ hipError_t launch_soa_to_aos(hipStream_t s, const void *soa, double *aos, int nx
                              int nz, int prec);
 hipError_t launch_sample(hipStream_t s, dnsg_ctx *h, const void *coarse_state, void *lo, void *up);
 hipError_t launch_interface(hipStream_t s, dnsg_ctx *h, int iter);
-hipError_t launch_ftcd(hipStream_t s, dnsg_ctx *h);
+hipError_t launch_ftcd(hipStream_t s, dnsg_ctx *h, double *psum = nullptr);
 // the same kernels over several solves of one precision (one launch per
 // BATCH_MAX items); the interface batch needs one gratio
 hipError_t launch_sample_many(hipStream_t s, dnsg_ctx *const *hs, int n);
 hipError_t launch_interface_many(hipStream_t s, dnsg_ctx *const *hs, int n, int iter);
-hipError_t launch_ftcd_many(hipStream_t s, dnsg_ctx *const *hs, int n);
+hipError_t launch_ftcd_many(hipStream_t s, dnsg_ctx *const *hs, int n, double *psum = nullptr);
+// workgroups of one injection launch (2 sides per handle), for the psum size
+int ftcd_blocks(const dnsg_ctx *h);
 // bulk momentum and density of each solve from its fused collide-stream sums:
 // out[2 slot[i]] = <rho u_x>, out[2 slot[i] + 1] = <rho> (encalc2.c:109-131 weights)
 hipError_t launch_flux_many(hipStream_t s, dnsg_ctx *const *hs, int n, double *out, const int *slot);
 hipError_t launch_encalc2(hipStream_t s, dnsg_ctx *h, double *diag_rec);
 hipError_t launch_encalc2_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec);
-hipError_t launch_wshear_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec);
+hipError_t launch_wshear_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec, const double *enc_rec = nullptr);
+// G of the next step from the destination-weighted sums of this step's writes
+hipError_t launch_flux_next(hipStream_t s, dnsg_ctx *h);
+size_t dsum_doubles(const dnsg_ctx *h);
+// encalc2 destination weights of one lattice (0 coarse, 1 lower band, 2 upper band)
+void set_dest_weights(const dnsg_ctx *h, int grid, LatArgs *L);
 hipError_t launch_wshear(hipStream_t s, dnsg_ctx *h, double *diag_rec);
 hipError_t launch_mass(hipStream_t s, dnsg_ctx *h, double *out);
 hipError_t launch_statistics(hipStream_t s, dnsg_ctx *h, double *out9nz);

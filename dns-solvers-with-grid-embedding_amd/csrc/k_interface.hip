@@ -414,19 +414,29 @@ struct FtcdItem {
     void *coarse;         // coarse post-stream lattice
     int Xc, Yc, GR;
     double fac;           // GR tau_c / tau_f
+    double *psum;         // if set: per workgroup [sum w e_x f, sum w f] of the values written (w = GR^3 / 2)
 };
 
-// item i: workgroups [0, nb) inject the lower band, [nb, 2 nb) the upper band
+// sums of two values over a 256-thread workgroup, valid in thread 0
+__device__ __forceinline__ void wg_sum2(double &a, double &b) {
+    __shared__ double l[2][4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_down(a, o, 64);
+        b += __shfl_down(b, o, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { l[0][w] = a; l[1][w] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = l[0][0] + l[0][1] + l[0][2] + l[0][3];
+        b = l[1][0] + l[1][1] + l[1][2] + l[1][3];
+    }
+}
+
 template <class S>
-__global__ __launch_bounds__(256) void k_ftcd(Batch<FtcdItem> B) {
-    const int it = batch_find(B.first, B.n, blockIdx.x);
-    const FtcdItem &A = B.it[it];
+__device__ __forceinline__ void ftcd_node(const FtcdItem &A, int side, int id, double &sm, double &sd) {
     const int Xc = A.Xc, Yc = A.Yc, GR = A.GR;
-    const int n = Xc * Yc, nb = (n + 255) / 256;
-    const int lb = blockIdx.x - B.first[it];
-    const int side = lb >= nb ? 1 : 0;
-    const int id = (lb - side * nb) * 256 + threadIdx.x;
-    if (id >= n) return;
     const int ic = id % Xc, jc = id / Xc;
     const int Xf = GR * Xc, Yf = GR * Yc;
     const double fac = A.fac;
@@ -445,7 +455,11 @@ __global__ __launch_bounds__(256) void k_ftcd(Batch<FtcdItem> B) {
             constexpr int q = decltype(qq)::value;
             const double feq = feq64<q>(m.dens, m.u0, m.u1, m.u2, m.v);
             const double fneq = f[q] - feq;
-            cs[lat_at(q, drow, ic, Xc)] = feq + fac * fneq;
+            const double out = feq + fac * fneq;
+            cs[lat_at(q, drow, ic, Xc)] = out;
+            sd += out;
+            if constexpr (EX[q] > 0) sm += out;
+            if constexpr (EX[q] < 0) sm -= out;
         });
     } else {
         const double drho = zeroth_moment(f), dens = 1. + drho;
@@ -455,12 +469,41 @@ __global__ __launch_bounds__(256) void k_ftcd(Batch<FtcdItem> B) {
             constexpr int q = decltype(qq)::value;
             const double eu = edot<q>(u0, u1, u2);
             const double geq = WQ[q] * (drho + dens * (3. * eu + 4.5 * eu * eu - 1.5 * v));
-            store_s(cs + lat_at(q, drow, ic, Xc), (float)(geq + fac * (f[q] - geq)));
+            const float out = (float)(geq + fac * (f[q] - geq));
+            store_s(cs + lat_at(q, drow, ic, Xc), out);
+            const double fv = read_back<S, q>(out);
+            sd += fv;
+            if constexpr (EX[q] > 0) sm += fv;
+            if constexpr (EX[q] < 0) sm -= fv;
         });
     }
 }
 
-hipError_t launch_ftcd_many(hipStream_t s, dnsg_ctx *const *hs, int n) {
+// item i: workgroups [0, nb) inject the lower band, [nb, 2 nb) the upper band
+template <class S>
+__global__ __launch_bounds__(256) void k_ftcd(Batch<FtcdItem> B) {
+    const int it = batch_find(B.first, B.n, blockIdx.x);
+    const FtcdItem &A = B.it[it];
+    const int Xc = A.Xc, Yc = A.Yc, GR = A.GR;
+    const int n = Xc * Yc, nb = (n + 255) / 256;
+    const int lb = blockIdx.x - B.first[it];
+    const int side = lb >= nb ? 1 : 0;
+    const int id = (lb - side * nb) * 256 + threadIdx.x;
+    double sm = 0., sd = 0.;  // weighted sums of the values written (flux controller)
+    if (id < n) ftcd_node<S>(A, side, id, sm, sd);
+    if (A.psum) {  // uniform per workgroup
+        const double w = 0.5 * GR * GR * GR;  // coarse end layers in encalc2 (encalc2.c:72-75)
+        sm *= w;
+        sd *= w;
+        wg_sum2(sm, sd);
+        if (threadIdx.x == 0) { A.psum[2 * lb] = sm; A.psum[2 * lb + 1] = sd; }
+    }
+}
+
+
+int ftcd_blocks(const dnsg_ctx *h) { return 2 * ((h->d.Xc * h->d.Yc + 255) / 256); }
+
+hipError_t launch_ftcd_many(hipStream_t s, dnsg_ctx *const *hs, int n, double *psum) {
     if (n < 1) return hipSuccess;
     std::vector<FtcdItem> v(n);
     for (int i = 0; i < n; i++) {
@@ -469,7 +512,8 @@ hipError_t launch_ftcd_many(hipStream_t s, dnsg_ctx *const *hs, int n) {
         // lower band k = zdf-1-GR is stored layer zdf-2-GR; upper band k = GR is stored layer GR;
         // GR*(tauf/tauc) with omega = 1/tau (ftcdtransfer.c:66)
         v[i] = {{h->lower.n(), h->upper.n()}, {zdf - 2 - GR, GR}, {0, h->d.zdc - 1}, h->coarse.n(),
-                h->d.Xc, h->d.Yc, GR, GR * (h->d.omega_f / h->d.omega_c)};
+                h->d.Xc, h->d.Yc, GR, GR * (h->d.omega_f / h->d.omega_c), psum};
+        if (psum) psum += 2 * (size_t)ftcd_blocks(h);
     }
     const int prec = hs[0]->p.precision;
     for (int i0 = 0; i0 < n; i0 += BATCH_MAX) {
@@ -488,6 +532,6 @@ hipError_t launch_ftcd_many(hipStream_t s, dnsg_ctx *const *hs, int n) {
     return hipSuccess;
 }
 
-hipError_t launch_ftcd(hipStream_t s, dnsg_ctx *h) { return launch_ftcd_many(s, &h, 1); }
+hipError_t launch_ftcd(hipStream_t s, dnsg_ctx *h, double *psum) { return launch_ftcd_many(s, &h, 1, psum); }
 
 }  // namespace dnsg

@@ -231,8 +231,94 @@ static hipError_t enc_two_stage(hipStream_t s, const EncArgs &A, double *seg) {
 
 size_t enc_seg_doubles(const dnsg_ctx *h) {
     const size_t total = (size_t)h->d.Yc * h->d.zdc * cs_chunks(h->d.Xc) +
-                         2 * (size_t)h->d.Yf * (h->d.zdf - 1) * cs_chunks(h->d.Xf);
+                         2 * (size_t)h->d.Yf * (h->d.zdf - 1) * cs_chunks(h->d.Xf) + (size_t)ftcd_blocks(h);
     return 4 * (total / ENC_SEG + 2);
+}
+
+// ---- flux controller from the writes of a step ------------------------------------------
+// The state at the top of step n+1 is written during step n by: the coarse
+// collide-stream (coarse layers 1..zdc-2), the injection (coarse layers 0, zdc-1),
+// the last band collide-stream (every band layer except the interface layer, which
+// carries no encalc2 weight).  Each writer sums its writes weighted by their
+// destination layer's encalc2 weight (encalc2.c:37-105), so G = ub - <rho u> of the
+// next step needs no pass over the lattices.
+
+size_t dsum_doubles(const dnsg_ctx *h) {
+    return 2 * ((size_t)h->d.Yc * h->d.zdc * cs_chunks(h->d.Xc) +
+                2 * (size_t)h->d.Yf * (h->d.zdf - 1) * cs_chunks(h->d.Xf));
+}
+
+void set_dest_weights(const dnsg_ctx *h, int grid, LatArgs *L) {
+    const int GR = h->d.GR, zdf = h->d.zdf, zds = zdf - 1;
+    if (grid == 0) { L->klo = 1; L->khi = h->d.zdc - 2; L->khalf = -1; L->kscale = 1. * GR * GR * GR; }
+    else if (grid == 1) { L->klo = 0; L->khi = zdf - GR - 2; L->khalf = -1; L->kscale = 1.; }  // k = 1 .. zdf-GR-1
+    else { L->klo = GR; L->khi = zds - 1; L->khalf = GR; L->kscale = 1.; }                      // k = GR .. zdf-2
+}
+
+struct FluxNextArgs {
+    const double *dc, *dl, *du, *fs;  // partial pairs: coarse, lower, upper launches; injection
+    int nc, nl, nu, nf;               // pairs in each
+    int Xf, Yf, NZs;
+    double ub;
+    double *force, *enc;
+};
+
+__global__ __launch_bounds__(RB) void k_flux_seg(FluxNextArgs A, double *seg) {
+    __shared__ double lds[4][RB / 64];
+    const int total = A.nc + A.nl + A.nu + A.nf;
+    const int e0 = blockIdx.x * ENC_SEG, e1 = min(total, e0 + ENC_SEG);
+    double v[4] = {0., 0., 0., 0.};  // momentum, density, coarse momentum, coarse density
+    for (int e = e0 + threadIdx.x; e < e1; e += RB) {
+        const double *p;
+        bool coarse = false;
+        if (e < A.nc) { p = A.dc + 2 * (size_t)e; coarse = true; }
+        else if (e < A.nc + A.nl) p = A.dl + 2 * (size_t)(e - A.nc);
+        else if (e < A.nc + A.nl + A.nu) p = A.du + 2 * (size_t)(e - A.nc - A.nl);
+        else { p = A.fs + 2 * (size_t)(e - A.nc - A.nl - A.nu); coarse = true; }
+        v[0] += p[0];
+        v[1] += p[1];
+        if (coarse) { v[2] += p[0]; v[3] += p[1]; }
+    }
+    block_sum<4>(v, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; k++) seg[blockIdx.x * 4 + k] = v[k];
+}
+
+__global__ __launch_bounds__(RB) void k_flux_fin(FluxNextArgs A, const double *seg, int nseg) {
+    __shared__ double lds[4][RB / 64];
+    double v[4] = {0., 0., 0., 0.};
+    for (int b = threadIdx.x; b < nseg; b += RB)
+        for (int k = 0; k < 4; k++) v[k] += seg[b * 4 + k];
+    block_sum<4>(v, lds);
+    if (threadIdx.x == 0) {
+        const double vol = ((double)((A.Xf + 1 - 1.) * (A.Yf + 1 - 1.) * (A.NZs - 2.)));  // encalc2.c:109
+        const double momentum = v[0] / vol, denst = v[1] / vol;
+        const double G = A.ub - momentum;                                               // :129
+        A.force[0] = G;
+        A.force[1] = v[2] / vol;
+        A.force[2] = v[3] / vol;
+        A.enc[0] = G;
+        A.enc[1] = momentum;
+        A.enc[2] = denst;
+        A.enc[3] = momentum * ((A.Yf) * (A.NZs - 2)) / denst;  // :138
+    }
+}
+
+hipError_t launch_flux_next(hipStream_t s, dnsg_ctx *h) {
+    FluxNextArgs A;
+    A.nc = h->d.Yc * h->d.zdc * cs_chunks(h->d.Xc);
+    A.nl = A.nu = h->d.Yf * (h->d.zdf - 1) * cs_chunks(h->d.Xf);
+    A.nf = ftcd_blocks(h);
+    A.dc = h->dsum;
+    A.dl = A.dc + 2 * (size_t)A.nc;
+    A.du = A.dl + 2 * (size_t)A.nl;
+    A.fs = h->fsum;
+    A.Xf = h->d.Xf; A.Yf = h->d.Yf; A.NZs = h->d.NZs; A.ub = h->p.ub;
+    A.force = h->d_force; A.enc = h->enc_next;
+    const int nseg = (A.nc + A.nl + A.nu + A.nf + ENC_SEG - 1) / ENC_SEG;
+    hipLaunchKernelGGL(k_flux_seg, dim3(nseg), dim3(RB), 0, s, A, h->seg);
+    hipLaunchKernelGGL(k_flux_fin, dim3(1), dim3(RB), 0, s, A, (const double *)h->seg, nseg);
+    return hipGetLastError();
 }
 
 static EncArgs enc_args(dnsg_ctx *h, const double *pc, const double *pl, const double *pu, int st, bool fused,
@@ -339,7 +425,8 @@ struct WsArgs {
     double omega_f;
     const double *force;
     double *diag;
-    double *out;  // [4]: str_up, str_down, stress, pgrad
+    double *out;        // [4]: str_up, str_down, stress, pgrad
+    const double *enc;  // if set: this step's encalc2 record (G, <rho u>, <rho>, flowrate) for diag[1..4]
 };
 
 __device__ void wshear_finish(const WsArgs &A, const double *v);
@@ -369,7 +456,11 @@ __global__ __launch_bounds__(RB) void k_fin_wshear_fused(WsArgs A, const double 
         v[3] += at(pu, nz - 3, y, c, 0);
     }
     block_sum<4>(v, lds);
-    if (threadIdx.x == 0) wshear_finish(A, v);
+    if (threadIdx.x == 0) {
+        wshear_finish(A, v);
+        if (A.enc && A.diag)
+            for (int k = 0; k < 4; k++) A.diag[1 + k] = A.enc[k];
+    }
 }
 
 __device__ void wshear_finish(const WsArgs &A, const double *v) {
@@ -388,13 +479,13 @@ __device__ void wshear_finish(const WsArgs &A, const double *v) {
     }
 }
 
-hipError_t launch_wshear_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec) {
+hipError_t launch_wshear_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec, const double *enc_rec) {
     const int cpr = cs_chunks(h->lower.nx);
     const double *pl = h->red + (size_t)h->coarse.ny * h->coarse.nz * cs_chunks(h->coarse.nx) * RED_W;
     const double *pu = pl + (size_t)h->lower.ny * h->lower.nz * cpr * RED_W;
     WsArgs A;
     A.part = nullptr; A.Xf = h->d.Xf; A.Yf = h->d.Yf; A.NZs = h->d.NZs; A.GR = h->d.GR; A.omega_f = h->d.omega_f;
-    A.force = h->d_force; A.diag = diag_rec; A.out = nullptr;
+    A.force = h->d_force; A.diag = diag_rec; A.out = nullptr; A.enc = enc_rec;
     hipLaunchKernelGGL(k_fin_wshear_fused, dim3(1), dim3(RB), 0, s, A, pl, pu, h->lower.nz, cpr);
     return hipGetLastError();
 }
@@ -412,7 +503,7 @@ hipError_t launch_wshear(hipStream_t s, dnsg_ctx *h, double *diag_rec) {
         hipLaunchKernelGGL(k_wshear_rows<__half>, grd, blk, 0, s, h->lower.c(), h->upper.c(), l.nx, l.ny, l.nz, part);
     WsArgs A;
     A.part = part; A.Xf = h->d.Xf; A.Yf = h->d.Yf; A.NZs = h->d.NZs; A.GR = h->d.GR; A.omega_f = h->d.omega_f;
-    A.force = h->d_force; A.diag = diag_rec; A.out = nullptr;
+    A.force = h->d_force; A.diag = diag_rec; A.out = nullptr; A.enc = nullptr;
     hipLaunchKernelGGL(k_fin_wshear, dim3(1), dim3(RB), 0, s, A);
     return hipGetLastError();
 }

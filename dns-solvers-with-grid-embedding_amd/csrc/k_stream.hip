@@ -68,9 +68,10 @@ __device__ __forceinline__ void store_cs(S *p, V v) {
     else store_s(p, v);
 }
 
-// fused-reduction sums of one thread's nodes (pre-collision moments)
+// fused-reduction sums of one thread's nodes: pre-collision moments (rho, mom, u)
+// and the destination-weighted sums of what it writes (dm, dd)
 struct RowSums {
-    double rho = 0., mom = 0., u = 0.;
+    double rho = 0., mom = 0., u = 0., dm = 0., dd = 0.;
     template <class S, class T>
     __device__ __forceinline__ void add(const T *f) {
         double d, m;
@@ -87,9 +88,22 @@ struct RowSums {
     }
 };
 
+// encalc2 weight of a destination layer (LatArgs::dsum)
+__device__ __forceinline__ double dest_w(const LatArgs &L, int zt) {
+    return (zt < L.klo || zt > L.khi) ? 0. : (zt == L.khalf ? 0.5 * L.kscale : L.kscale);
+}
+
+// add a store of direction d (e_x = ex) and f-value fv into a layer of weight W
+__device__ __forceinline__ void add_write(RowSums &rs, double W, int ex, double fv) {
+    const double a = W * fv;
+    rs.dd += a;
+    if (ex > 0) rs.dm += a;
+    else if (ex < 0) rs.dm -= a;
+}
+
 // Scatter form for the node at x of row (y, z): collide, then write each
 // direction to its target (x + e_x, y + e_y, z + e_z) with the wall rules.
-template <class S, class T>
+template <class S, bool DS, class T>
 __device__ __forceinline__ void cs_scatter(const LatArgs &L, const S *__restrict__ cur, S *__restrict__ nxt, int row,
                                            int x, T G, T om, bool red, RowSums &rs) {
     const int nx = L.nx, ny = L.ny, nz = L.nz;
@@ -103,6 +117,8 @@ __device__ __forceinline__ void cs_scatter(const LatArgs &L, const S *__restrict
     });
     if (red) rs.add<S>(f);
     collide_any(f, G, om);
+    const bool ds = DS && L.dsum != nullptr;  // DS: compiled in only where a launch asks for it
+    const double wz[3] = {ds ? dest_w(L, z - 1) : 0., ds ? dest_w(L, z) : 0., ds ? dest_w(L, z + 1) : 0.};
     const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
     unroll<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
@@ -116,10 +132,13 @@ __device__ __forceinline__ void cs_scatter(const LatArgs &L, const S *__restrict
                     const int8_t *wc = (EZ[q] < 0) ? L.wlo : L.whi;
                     const int sld = wc[(size_t)yt * nx + xt];
                     const int sld1 = wc[(size_t)y * nx + x];
-                    if (sld == -1 && sld1 == -1)
+                    if (sld == -1 && sld1 == -1) {
                         store_s(nxt + lat_at(OPP[q], (size_t)row, x, nx), f[q]);
-                    else
+                        if (ds) add_write(rs, wz[1], -EX[q], read_back<S, q>(f[q]));
+                    } else {
                         store_s(nxt + lat_at(MIR[q], (size_t)z * ny + yt, xt, nx), f[q]);
+                        if (ds) add_write(rs, wz[1], EX[q], read_back<S, q>(f[q]));
+                    }
                 }
                 return;
             }
@@ -127,6 +146,7 @@ __device__ __forceinline__ void cs_scatter(const LatArgs &L, const S *__restrict
         const int zt = z + EZ[q];
         if ((zt == 0 && L.skip_lo) || (zt == nz - 1 && L.skip_hi)) return;
         store_s(nxt + lat_at(q, (size_t)zt * ny + yt, xt, nx), f[q]);
+        if (ds) add_write(rs, wz[EZ[q] + 1], EX[q], read_back<S, q>(f[q]));
     });
 }
 
@@ -137,7 +157,7 @@ __device__ __forceinline__ void cs_scatter(const LatArgs &L, const S *__restrict
 // positions of the target row: whole cache lines, written once.  Each thread
 // stores, at its own x, the value arriving there from x - e_x; the set of writes
 // is exactly the scatter form's.
-template <class S, class T>
+template <class S, bool DS, class T>
 __device__ __forceinline__ void cs_aligned_chunk(const LatArgs &L, const S *__restrict__ cur, S *__restrict__ nxt,
                                                  int row, int x0, T G, T om, bool red, RowSums &rs) {
     constexpr int NW = CS_THREADS / 64;
@@ -179,6 +199,8 @@ __device__ __forceinline__ void cs_aligned_chunk(const LatArgs &L, const S *__re
         }
     });
     __syncthreads();
+    const bool ds = DS && L.dsum != nullptr;  // DS: compiled in only where a launch asks for it
+    const double wz[3] = {ds ? dest_w(L, z - 1) : 0., ds ? dest_w(L, z) : 0., ds ? dest_w(L, z + 1) : 0.};
     const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
     unroll<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
@@ -199,12 +221,16 @@ __device__ __forceinline__ void cs_aligned_chunk(const LatArgs &L, const S *__re
                     const int8_t *wc = (EZ[q] < 0) ? L.wlo : L.whi;
                     // specular reflection of the source x - e_x into (x, yt), unless it bounces back
                     const int xs = (EX[q] > 0) ? xm : (EX[q] < 0 ? xp : x);
-                    if (!(wc[(size_t)yt * nx + x] == -1 && wc[(size_t)y * nx + xs] == -1))
+                    if (!(wc[(size_t)yt * nx + x] == -1 && wc[(size_t)y * nx + xs] == -1)) {
                         store_cs(nxt + lat_at(MIR[q], (size_t)z * ny + yt, x, nx), v);
+                        if (ds) add_write(rs, wz[1], EX[q], read_back<S, q>(v));
+                    }
                     // halfway bounce-back of this node's own value
                     const int xt = (EX[q] > 0) ? xp : (EX[q] < 0 ? xm : x);
-                    if (wc[(size_t)yt * nx + xt] == -1 && wc[(size_t)y * nx + x] == -1)
+                    if (wc[(size_t)yt * nx + xt] == -1 && wc[(size_t)y * nx + x] == -1) {
                         store_cs(nxt + lat_at(OPP[q], (size_t)row, x, nx), f[q]);
+                        if (ds) add_write(rs, wz[1], -EX[q], read_back<S, q>(f[q]));
+                    }
                 }
                 return;
             }
@@ -212,6 +238,7 @@ __device__ __forceinline__ void cs_aligned_chunk(const LatArgs &L, const S *__re
         const int zt = z + EZ[q];
         if ((zt == 0 && L.skip_lo) || (zt == nz - 1 && L.skip_hi)) return;
         store_cs(nxt + lat_at(q, (size_t)zt * ny + yt, x, nx), v);
+        if (ds) add_write(rs, wz[EZ[q] + 1], EX[q], read_back<S, q>(v));
     });
 }
 
@@ -223,20 +250,26 @@ __device__ __forceinline__ void write_sums(double *red, size_t e, const RowSums 
 
 // workgroup sum of one partial (wave shuffles, then LDS over the waves)
 __device__ __forceinline__ RowSums block_sums(RowSums s) {
-    __shared__ double lds[3][CS_THREADS / 64];
+    __shared__ double lds[5][CS_THREADS / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     s.rho = wave_sum64(s.rho);
     s.mom = wave_sum64(s.mom);
     s.u = wave_sum64(s.u);
-    if (lane == 0) { lds[0][wid] = s.rho; lds[1][wid] = s.mom; lds[2][wid] = s.u; }
+    s.dm = wave_sum64(s.dm);
+    s.dd = wave_sum64(s.dd);
+    if (lane == 0) {
+        lds[0][wid] = s.rho; lds[1][wid] = s.mom; lds[2][wid] = s.u; lds[3][wid] = s.dm; lds[4][wid] = s.dd;
+    }
     __syncthreads();
     RowSums t;
     if (threadIdx.x == 0)
-        for (int w = 0; w < CS_THREADS / 64; w++) { t.rho += lds[0][w]; t.mom += lds[1][w]; t.u += lds[2][w]; }
+        for (int w = 0; w < CS_THREADS / 64; w++) {
+            t.rho += lds[0][w]; t.mom += lds[1][w]; t.u += lds[2][w]; t.dm += lds[3][w]; t.dd += lds[4][w];
+        }
     return t;
 }
 
-template <class S>
+template <class S, bool DS>
 __global__ __launch_bounds__(CS_THREADS) void k_collide_stream(Batch<LatArgs> P) {
     using T = typename ComputeOf<S>::T;
     const int bg = xcd_remap(blockIdx.x, gridDim.x);
@@ -257,25 +290,31 @@ __global__ __launch_bounds__(CS_THREADS) void k_collide_stream(Batch<LatArgs> P)
         const int row = b / cpr, chunk = b % cpr;
         const S *__restrict__ cur = base + (size_t)row * Q * nx;
         if (cs_aligned<S>() && nx % CS_THREADS == 0) {
-            cs_aligned_chunk<S, T>(L, cur, nxt, row, chunk * CS_THREADS, G, om, red, rs);
+            cs_aligned_chunk<S, DS, T>(L, cur, nxt, row, chunk * CS_THREADS, G, om, red, rs);
         } else {
             const int x = chunk * CS_THREADS + threadIdx.x;
-            if (x < nx) cs_scatter<S, T>(L, cur, nxt, row, x, G, om, red, rs);  // a partial chunk still sums below
+            if (x < nx) cs_scatter<S, DS, T>(L, cur, nxt, row, x, G, om, red, rs);  // a partial chunk still sums below
         }
-        if (red) {  // uniform per workgroup
+        if (red || (DS && L.dsum)) {  // uniform per workgroup
             const RowSums t = block_sums(rs);
-            if (threadIdx.x == 0) write_sums(L.red, ((size_t)row * cpr + chunk) * RED_W, t);
+            const size_t e = (size_t)row * cpr + chunk;
+            if (threadIdx.x == 0 && red) write_sums(L.red, e * RED_W, t);
+            if (DS && threadIdx.x == 0 && L.dsum) { L.dsum[2 * e] = t.dm; L.dsum[2 * e + 1] = t.dd; }
         }
     } else {
         // narrow lattice: rpw whole rows per workgroup, one partial per row summed in x order
         const int r = threadIdx.x / nx, x = threadIdx.x - r * nx;
         const int row = b * rpw + r;
-        if (r < rpw && row < ny * L.nz) cs_scatter<S, T>(L, base + (size_t)row * Q * nx, nxt, row, x, G, om, red, rs);
-        if (red) {
-            __shared__ double lr[3][CS_THREADS];
+        if (r < rpw && row < ny * L.nz) cs_scatter<S, DS, T>(L, base + (size_t)row * Q * nx, nxt, row, x, G, om, red, rs);
+        if (red || (DS && L.dsum)) {
+            __shared__ double lr[DS ? 5 : 3][CS_THREADS];
             lr[0][threadIdx.x] = rs.rho;
             lr[1][threadIdx.x] = rs.mom;
             lr[2][threadIdx.x] = rs.u;
+            if constexpr (DS) {
+                lr[3][threadIdx.x] = rs.dm;
+                lr[4][threadIdx.x] = rs.dd;
+            }
             __syncthreads();
             const int rr = b * rpw + (int)threadIdx.x;
             if ((int)threadIdx.x < rpw && rr < ny * L.nz) {
@@ -284,8 +323,13 @@ __global__ __launch_bounds__(CS_THREADS) void k_collide_stream(Batch<LatArgs> P)
                     t.rho += lr[0][k];
                     t.mom += lr[1][k];
                     t.u += lr[2][k];
+                    if constexpr (DS) {
+                        t.dm += lr[3][k];
+                        t.dd += lr[4][k];
+                    }
                 }
-                write_sums(L.red, (size_t)rr * RED_W, t);
+                if (red) write_sums(L.red, (size_t)rr * RED_W, t);
+                if (DS && L.dsum) { L.dsum[2 * (size_t)rr] = t.dm; L.dsum[2 * (size_t)rr + 1] = t.dd; }
             }
         }
     }
@@ -303,12 +347,19 @@ hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, i
         }
         P.first[BATCH_MAX] = blocks;
         if (blocks == 0) continue;
-        if (prec == DNSG_PREC_FP64)
-            hipLaunchKernelGGL(k_collide_stream<double>, dim3(blocks), dim3(CS_THREADS), 0, s, P);
-        else if (prec == DNSG_PREC_FP32)
-            hipLaunchKernelGGL(k_collide_stream<float>, dim3(blocks), dim3(CS_THREADS), 0, s, P);
-        else
-            hipLaunchKernelGGL(k_collide_stream<__half>, dim3(blocks), dim3(CS_THREADS), 0, s, P);
+        bool ds = false;
+        for (int i = 0; i < P.n; i++) ds = ds || P.it[i].dsum != nullptr;
+        const dim3 g(blocks), b(CS_THREADS);
+        if (prec == DNSG_PREC_FP64) {
+            if (ds) hipLaunchKernelGGL((k_collide_stream<double, true>), g, b, 0, s, P);
+            else hipLaunchKernelGGL((k_collide_stream<double, false>), g, b, 0, s, P);
+        } else if (prec == DNSG_PREC_FP32) {
+            if (ds) hipLaunchKernelGGL((k_collide_stream<float, true>), g, b, 0, s, P);
+            else hipLaunchKernelGGL((k_collide_stream<float, false>), g, b, 0, s, P);
+        } else {
+            if (ds) hipLaunchKernelGGL((k_collide_stream<__half, true>), g, b, 0, s, P);
+            else hipLaunchKernelGGL((k_collide_stream<__half, false>), g, b, 0, s, P);
+        }
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

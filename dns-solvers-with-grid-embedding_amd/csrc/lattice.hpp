@@ -161,6 +161,14 @@ __device__ __forceinline__ double load_s(const double *p) { return *p; }
 __device__ __forceinline__ void store_s(float *p, float v) { *p = v; }
 __device__ __forceinline__ void store_s(__half *p, float v) { *p = __float2half(v); }
 __device__ __forceinline__ void store_s(double *p, double v) { *p = v; }
+// the value a later read of a store returns, as f in FP64 (the shifted tiers
+// store g = f - w, FP16 after rounding to half)
+template <class S, int q, class T>
+__device__ __forceinline__ double read_back(T v) {
+    if constexpr (sizeof(S) == 8) return (double)v;
+    else if constexpr (sizeof(S) == 4) return (double)(float)v + WQ[q];
+    else return (double)__half2float(__float2half((float)v)) + WQ[q];
+}
 // streaming stores that bypass the caches' normal allocation (written once, read next step)
 __device__ __forceinline__ void store_nt(float *p, float v) { __builtin_nontemporal_store(v, p); }
 __device__ __forceinline__ void store_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
