@@ -198,6 +198,17 @@ __device__ __forceinline__ P edot_rt(int q, P u0, P u1, P u2) {
     return s;
 }
 
+// two adjacent values of one row, one vector store (the first at an even index)
+__device__ __forceinline__ void store_pair(double *p, double a, double b) {
+    *reinterpret_cast<double2 *>(p) = make_double2(a, b);
+}
+__device__ __forceinline__ void store_pair(float *p, float a, float b) {
+    *reinterpret_cast<float2 *>(p) = make_float2(a, b);
+}
+__device__ __forceinline__ void store_pair(__half *p, float a, float b) {
+    *reinterpret_cast<__half2 *>(p) = __floats2half2_rn(a, b);
+}
+
 // ---- fused interface fill ----------------------------------------------------------
 // One workgroup = one TX x TY tile of coarse cells of one band.  For each of the
 // 23 plane values in turn it
@@ -221,13 +232,14 @@ struct IfaceItem {
 
 template <class S, class P, int GR, int TX, int TY>
 __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<P> H) {
-    constexpr int NT = TX * TY, HX = TX + 5, HY = TY + 5, NP = GR * GR, FX = GR * TX, FY = GR * TY;
+    constexpr int NT = TX * TY, HX = TX + 5, HY = TY + 5, NP = GR * GR;
     constexpr int NH = (HX * HY + NT - 1) / NT;  // halo points per thread
-    __shared__ P s0[HY][HX];          // value, halo 2 on each side (+1 corner)
-    __shared__ P s1[TY + 1][TX + 1];  // d/dx at the corners
-    __shared__ P s2[TY + 1][HX];      // d/dy at the corners and the halo columns
-    __shared__ P s3[TY + 1][TX + 1];  // d2/dxdy at the corners
-    __shared__ P fo[FY][FX];          // output tile of one direction
+    // two sets, used by alternate values: the next value's tiles are written while
+    // slower threads still read this value's
+    __shared__ P s0[2][HY][HX];          // value, halo 2 on each side (+1 corner)
+    __shared__ P s1[2][TY + 1][TX + 1];  // d/dx at the corners
+    __shared__ P s2[2][TY + 1][HX];      // d/dy at the corners and the halo columns
+    __shared__ P s3[2][TY + 1][TX + 1];  // d2/dxdy at the corners
     const int itm = batch_find(BT.first, BT.n, blockIdx.x);
     const IfaceItem &B = BT.it[itm];
     const int lbg = blockIdx.x - BT.first[itm];
@@ -261,6 +273,7 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
     P dens[NP], u0[NP], u1[NP], u2[NP], vv[NP];
     for (int iv = 0; iv < nv; iv++) {
         const int v = iv < 4 ? iv : 4 + q0 + (iv - 4);
+        const int sb = iv & 1;
         // 1. Hermite value on the halo tile (tinterpolate_hrmt.c:28-34)
 #pragma unroll
         for (int k = 0; k < NH; k++) {
@@ -271,7 +284,7 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
             const P c2 = (ln - lpp) * (P)0.5;
             const P c3 = ln;
             const P c4 = ((P)3. * ln - (P)4. * lp + lpp) * (P)0.5;
-            s0[p / HX][p % HX] = H.h00 * c1 + H.h10 * c2 + H.h01 * c3 + H.h11 * c4;
+            s0[sb][p / HX][p % HX] = H.h00 * c1 + H.h10 * c2 + H.h01 * c3 + H.h11 * c4;
         }
         if (iv + 1 < nv) {  // prefetch the next value
             const size_t vo = (size_t)(iv + 1 < 4 ? iv + 1 : 4 + q0 + (iv + 1 - 4)) * n;
@@ -281,27 +294,33 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
         }
         __syncthreads();
         // 2. d/dy on corner rows for every halo column; d/dx at the corners (derivative.c:118-119)
+        P(*t0)[HX] = s0[sb];
         for (int p = tid; p < (TY + 1) * HX; p += NT) {
             const int r = p / HX, hx = p % HX, hy = r + 2;
-            s2[r][hx] = d4(s0[hy - 2][hx], s0[hy - 1][hx], s0[hy + 1][hx], s0[hy + 2][hx]);
+            s2[sb][r][hx] = d4(t0[hy - 2][hx], t0[hy - 1][hx], t0[hy + 1][hx], t0[hy + 2][hx]);
             if (hx >= 2 && hx <= TX + 2)
-                s1[r][hx - 2] = d4(s0[hy][hx - 2], s0[hy][hx - 1], s0[hy][hx + 1], s0[hy][hx + 2]);
+                s1[sb][r][hx - 2] = d4(t0[hy][hx - 2], t0[hy][hx - 1], t0[hy][hx + 1], t0[hy][hx + 2]);
         }
-        __syncthreads();
-        // 3. d2/dxdy = d/dx of d/dy (derivative.c:139)
+        // 3. d2/dxdy = d/dx of d/dy (derivative.c:139); the d/dy values are formed
+        //    again here, with the same expression, so no wait for the loop above
         for (int p = tid; p < (TY + 1) * (TX + 1); p += NT) {
-            const int r = p / (TX + 1), c = p % (TX + 1), hx = c + 2;
-            s3[r][c] = d4(s2[r][hx - 2], s2[r][hx - 1], s2[r][hx + 1], s2[r][hx + 2]);
+            const int r = p / (TX + 1), c = p % (TX + 1), hx = c + 2, hy = r + 2;
+            auto dy = [&](int h) { return d4(t0[hy - 2][h], t0[hy - 1][h], t0[hy + 1][h], t0[hy + 2][h]); };
+            s3[sb][r][c] = d4(dy(hx - 2), dy(hx - 1), dy(hx + 1), dy(hx + 2));
         }
         __syncthreads();
         // 4. bicubic patch of this cell (bcucof_opt.c:47-61) at its fine columns
         P r[NP];
         {
             P xv[16], c[16];
-            xv[0] = s0[cy + 2][cx + 2]; xv[4] = s1[cy][cx];         xv[8] = s2[cy][cx + 2];      xv[12] = s3[cy][cx];
-            xv[1] = s0[cy + 2][cx + 3]; xv[5] = s1[cy][cx + 1];     xv[9] = s2[cy][cx + 3];      xv[13] = s3[cy][cx + 1];
-            xv[2] = s0[cy + 3][cx + 3]; xv[6] = s1[cy + 1][cx + 1]; xv[10] = s2[cy + 1][cx + 3]; xv[14] = s3[cy + 1][cx + 1];
-            xv[3] = s0[cy + 3][cx + 2]; xv[7] = s1[cy + 1][cx];     xv[11] = s2[cy + 1][cx + 2]; xv[15] = s3[cy + 1][cx];
+            const P(*u0_)[HX] = s0[sb];
+            const P(*u1_)[TX + 1] = s1[sb];
+            const P(*u2_)[HX] = s2[sb];
+            const P(*u3_)[TX + 1] = s3[sb];
+            xv[0] = u0_[cy + 2][cx + 2]; xv[4] = u1_[cy][cx];         xv[8] = u2_[cy][cx + 2];      xv[12] = u3_[cy][cx];
+            xv[1] = u0_[cy + 2][cx + 3]; xv[5] = u1_[cy][cx + 1];     xv[9] = u2_[cy][cx + 3];      xv[13] = u3_[cy][cx + 1];
+            xv[2] = u0_[cy + 3][cx + 3]; xv[6] = u1_[cy + 1][cx + 1]; xv[10] = u2_[cy + 1][cx + 3]; xv[14] = u3_[cy + 1][cx + 1];
+            xv[3] = u0_[cy + 3][cx + 2]; xv[7] = u1_[cy + 1][cx];     xv[11] = u2_[cy + 1][cx + 2]; xv[15] = u3_[cy + 1][cx];
             bicubic_coef(xv, c);
 #pragma unroll
             for (int pp = 0; pp < NP; pp++) {
@@ -321,32 +340,37 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
 #pragma unroll
                 for (int pp = 0; pp < NP; pp++) vv[pp] = u0[pp] * u0[pp] + u1[pp] * u1[pp] + u2[pp] * u2[pp];
             }
-            __syncthreads();  // s0..s3 are rewritten by the next value
             continue;
         }
         const int q = v - 4;
         const P wq = (P)c_WQ[q];
+        P out[NP];
 #pragma unroll
         for (int pp = 0; pp < NP; pp++) {
-            const int ii = pp / GR, jj = pp % GR;
             const P eu = edot_rt(q, u0[pp], u1[pp], u2[pp]);
-            P out;
             if constexpr (sizeof(S) == 8) {
                 const P feq = wq * dens[pp] * ((P)1. + (P)3. * eu + (P)4.5 * eu * eu - (P)1.5 * vv[pp]);
-                out = feq + rescale * r[pp];  // ctfdtransfer_opt.c:93
+                out[pp] = feq + rescale * r[pp];  // ctfdtransfer_opt.c:93
             } else {  // dens holds rho - 1
                 const P rho = (P)1. + dens[pp];
                 const P geq = wq * (dens[pp] + rho * ((P)3. * eu + (P)4.5 * eu * eu - (P)1.5 * vv[pp]));
-                out = geq + rescale * r[pp];
+                out[pp] = geq + rescale * r[pp];
             }
-            fo[GR * cy + jj][GR * cx + ii] = out;
         }
-        __syncthreads();
-        for (int p = tid; p < FY * FX; p += NT) {
-            const int fy = GR * y0 + p / FX, fx = GR * x0 + p % FX;
-            if (fx < Xf && fy < Yf) store_s(band + lat_at(q, (size_t)B.layer * Yf + fy, fx, Xf), fo[p / FX][p % FX]);
+        // the fine columns of this cell, straight from registers: a half-wave's cells
+        // cover one contiguous run of each fine row
+        if (x0 + cx < Xc && y0 + cy < Yc) {
+#pragma unroll
+            for (int jj = 0; jj < GR; jj++) {
+                S *dst = band + lat_at(q, (size_t)B.layer * Yf + GR * (y0 + cy) + jj, GR * (x0 + cx), Xf);
+                if constexpr (GR == 2) {
+                    store_pair(dst, out[jj], out[GR + jj]);  // ii = 0, 1
+                } else {
+#pragma unroll
+                    for (int ii = 0; ii < GR; ii++) store_s(dst + ii, out[ii * GR + jj]);
+                }
+            }
         }
-        __syncthreads();
     }
 }
 
