@@ -374,11 +374,14 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
     }
 }
 
-// coarse tile of one workgroup, and workgroups per tile (the 19 directions can be
-// shared out over several workgroups, each forming rho and u itself; measured on
-// the Re_tau=180 bands, one group is fastest: the fill is bound by the per-value
-// work, which the split multiplies by (4 + 19/G) G / 23)
-constexpr int IF_TX = 32, IF_TY = 8, IF_GROUPS = 1;
+// Coarse tile of one workgroup.  The 19 directions can be shared out over G
+// workgroups per tile, each forming rho and u itself: that multiplies the work by
+// (4 + 19/G) G / 23 but shortens each workgroup's chain of sequential value phases
+// to 4 + 19/G.  A launch with enough tiles to fill the GPU (the Re_tau=180 bands:
+// 1024 tiles) is fastest with G = 1; the serving queue's small channels leave most
+// CUs idle and finish sooner with G = 4.
+constexpr int IF_TX = 32, IF_TY = 8;
+static int iface_groups(long tiles) { return tiles >= 1024 ? 1 : (tiles >= 512 ? 2 : 4); }
 
 template <class S, class P>
 static hipError_t iface_items(hipStream_t s, int GR, const IfaceItem *items, int n, HermiteW<P> H) {
@@ -406,8 +409,8 @@ static void iface_pair(const dnsg_ctx *h, IfaceItem *out) {
     // tauc/((double)GR*tauf) (ctfdtransfer_opt.c:93)
     const double rescale = h->d.omega_c / ((double)h->d.GR * h->d.omega_f);
     const int zds = h->upper.nz;
-    out[0] = {h->lev_lo[0], h->lev_lo[1], h->lev_lo[2], h->lower.n(), zds - 1, h->d.Xc, h->d.Yc, IF_GROUPS, rescale};
-    out[1] = {h->lev_up[0], h->lev_up[1], h->lev_up[2], h->upper.n(), 0, h->d.Xc, h->d.Yc, IF_GROUPS, rescale};
+    out[0] = {h->lev_lo[0], h->lev_lo[1], h->lev_lo[2], h->lower.n(), zds - 1, h->d.Xc, h->d.Yc, 1, rescale};
+    out[1] = {h->lev_up[0], h->lev_up[1], h->lev_up[2], h->upper.n(), 0, h->d.Xc, h->d.Yc, 1, rescale};
 }
 
 hipError_t launch_interface_many(hipStream_t s, dnsg_ctx *const *hs, int n, int iter) {
@@ -421,7 +424,13 @@ hipError_t launch_interface_many(hipStream_t s, dnsg_ctx *const *hs, int n, int 
     H.h01 = (-2. * tmp * tmp * tmp + 3. * tmp * tmp);
     H.h11 = (tmp * tmp * tmp - tmp * tmp);
     std::vector<IfaceItem> v(2 * (size_t)n);
-    for (int i = 0; i < n; i++) iface_pair(hs[i], &v[2 * (size_t)i]);
+    long tiles = 0;
+    for (int i = 0; i < n; i++) {
+        iface_pair(hs[i], &v[2 * (size_t)i]);
+        tiles += 2L * ((hs[i]->d.Xc + IF_TX - 1) / IF_TX) * ((hs[i]->d.Yc + IF_TY - 1) / IF_TY);
+    }
+    const int G = iface_groups(tiles);
+    for (auto &it : v) it.groups = G;
     if (prec == DNSG_PREC_FP64) return iface_items<double, double>(s, GR, v.data(), 2 * n, H);
     HermiteW<float> Hf = {(float)H.h00, (float)H.h10, (float)H.h01, (float)H.h11};
     if (prec == DNSG_PREC_FP32) return iface_items<float, float>(s, GR, v.data(), 2 * n, Hf);
