@@ -110,6 +110,7 @@ SIGNATURES = {
     "dnsg_stats_average": [_P, _DP, ctypes.POINTER(ctypes.c_long)],
     "dnsg_stats_reset": [_P],
     "dnsg_velocity_snapshot": [_P, ctypes.c_int, _DP, _DP],
+    "dnsg_spectra": [_P, ctypes.c_int, _DP, _DP],
     "dnsg_queue_opts_default": [ctypes.POINTER(QueueOpts)],
     "dnsg_queue_create": [ctypes.POINTER(QueueOpts), ctypes.POINTER(_P)],
     "dnsg_queue_destroy": [_P],
@@ -283,6 +284,16 @@ class Solver:
         return vel, den
 
     # reference-granular entry points
+    def spectra(self, grid):
+        """1-D energy spectra of grid 0/1/2: (ex [3][N0][nz], ey [3][N1/2+1][nz])"""
+        d = self.dims
+        N0, N1 = (d.Xc, d.Yc) if grid == 0 else (d.Xf, d.Yf)
+        nz = d.zdc if grid == 0 else d.zdf
+        ex = np.empty((3, N0, nz))
+        ey = np.empty((3, N1 // 2 + 1, nz))
+        self._check(self.lib.dnsg_spectra(self.h, grid, _ptr(ex), _ptr(ey)))
+        return ex, ey
+
     def encalc2(self, ss):
         d = Diag()
         self._check(self.lib.dnsg_encalc2(self.h, ss, ctypes.byref(d)))

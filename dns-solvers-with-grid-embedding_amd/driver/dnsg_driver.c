@@ -9,7 +9,9 @@
  * ITER switches) and/or key=value arguments with the dnsg_params field names,
  * which also reach the runtime-only extensions (force_mode, g_const, tau_c,
  * precision, init_kind, seed, u_max, noise, results_dat).  print_params=1 prints
- * the parsed parameters and the derived sizes and exits without GPU work.
+ * the parsed parameters and the derived sizes and exits without GPU work;
+ * spectra=1 also writes the 1-D energy spectra of every velocity snapshot (the
+ * files of the reference's e-spectra tool, computed on the device).
  *
  * Run modes (main:944-1190): -s starts from rest ('-s' of the reference) or,
  * with init_kind=1, from the seeded laminar profile with noise; -p reads the
@@ -40,6 +42,7 @@ static const double E[3][19] = {{0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, 1, -1, -
 
 static int results_dat = 1;
 static int print_params = 0;
+static int spectra_out = 0;
 
 static double now(void) {
     struct timespec t;
@@ -65,6 +68,7 @@ static int set_param(dnsg_params *p, const char *key, const char *val) {
     if (!strcmp(key, "seed")) { p->seed = strtoull(val, NULL, 10); return 0; }
     if (!strcmp(key, "results_dat")) { results_dat = atoi(val); return 0; }
     if (!strcmp(key, "print_params")) { print_params = atoi(val); return 0; }
+    if (!strcmp(key, "spectra")) { spectra_out = atoi(val); return 0; }
     if (!strcmp(key, "XDIM") || !strcmp(key, "YDIM")) return 0; /* one GPU = one rank */
     return -1;
 }
@@ -157,6 +161,30 @@ static void vel_print(dnsg_t *h, const dnsg_dims *d, long cntr) {
         snprintf(fn, sizeof fn, "vel-%s.%.4d", tag[g], (int)cntr); write_file(fn, vel, n * 3, NULL, 0);
         snprintf(fn, sizeof fn, "den-%s.%.4d", tag[g], (int)cntr); write_file(fn, den, n, NULL, 0);
         free(vel); free(den);
+    }
+}
+
+/* e-spectra/src/1d-spectra-mgrd.c:228-265 file part: the 1-D spectra of the three
+   grids from the current state, files E{uu,vv,ww}{x,y}-{c,fl,fu}.NNNN */
+static void spectra_print(dnsg_t *h, const dnsg_dims *d, long cntr) {
+    static const char *tag[3] = {"c", "fl", "fu"};
+    static const char *comp[3] = {"uu", "vv", "ww"};
+    int g, c;
+    for (g = 0; g < 3; g++) {
+        const int N0 = g == 0 ? d->Xc : d->Xf, N1h = (g == 0 ? d->Yc : d->Yf) / 2 + 1;
+        const int nz = g == 0 ? d->zdc : d->zdf;
+        double *ex = malloc(sizeof(double) * 3 * N0 * nz), *ey = malloc(sizeof(double) * 3 * N1h * nz);
+        char fn[64];
+        int rc;
+        if (!ex || !ey) { fprintf(stderr, "spectra: out of memory\n"); exit(1); }
+        if ((rc = dnsg_spectra(h, g, ex, ey))) die(h, "dnsg_spectra", rc);
+        for (c = 0; c < 3; c++) {
+            snprintf(fn, sizeof fn, "E%sx-%s.%.4d", comp[c], tag[g], (int)cntr);
+            write_file(fn, ex + (size_t)c * N0 * nz, (size_t)N0 * nz, NULL, 0);
+            snprintf(fn, sizeof fn, "E%sy-%s.%.4d", comp[c], tag[g], (int)cntr);
+            write_file(fn, ey + (size_t)c * N1h * nz, (size_t)N1h * nz, NULL, 0);
+        }
+        free(ex); free(ey);
     }
 }
 
@@ -404,6 +432,7 @@ int main(int argc, char **argv) {
             if ((rc = dnsg_encalc2(h, ss, &d0))) die(h, "dnsg_encalc2", rc);
             if (!(ss % (50 * ut))) checkpoint(h, &d, TM + (ss / ut), d0.force);
             vel_print(h, &d, TM + (ss / ut));
+            if (spectra_out) spectra_print(h, &d, TM + (ss / ut));
             statistics(h, &d, &p, TM + (ss / ut));
         }
         /* run up to the next output step (or the end) in one batch */

@@ -245,6 +245,17 @@ int dnsg_stats_reset(dnsg_t *h);
  * 1 = lower band, 2 = upper band; vel is [X][Y][Z][3], den is [X][Y][Z]. */
 int dnsg_velocity_snapshot(dnsg_t *h, int grid, double *vel, double *den);
 
+/* e-spectra/src/1d-spectra-mgrd.c:119-230 (the reference's offline spectra tool, here
+ * on the device from the current state): 1-D energy spectra of the velocity planes of
+ * one grid, grid 0 = coarse, 1 = lower band, 2 = upper band.  Every wall-normal plane
+ * of the grid's velocity file (vel_print.c; the bands include their wall layer) is
+ * transformed over (x, y), scaled by 1/(N0 N1), its mean mode removed, and folded:
+ *   ex [3][N0][nz]      E_uu, E_vv, E_ww over the streamwise wavenumber index
+ *                        (the files Euux/Evvx/Ewwx-{c,fl,fu}.NNNN)
+ *   ey [3][N1/2+1][nz]  the same over the spanwise wavenumber index (Euuy/Evvy/Ewwy)
+ * with N0, N1 the grid's streamwise and spanwise extents, nz = zdc or zdf. */
+int dnsg_spectra(dnsg_t *h, int grid, double *ex, double *ey);
+
 /* ---- serving queue: steady slip-length solves, continuously batched -----------
  * No counterpart in the reference, which runs one channel per MPI job.  It is
  * for parametric sweeps: many independent channels of uneven sizes, each driven

@@ -169,6 +169,31 @@ def test_driver_runs_from_definitions_file(tmp_path, oracle_built):
     _compare(odir, gdir, exact_state=False, tol=1e-11)
 
 
+@pytest.mark.gpu
+def test_driver_writes_spectra(tmp_path, oracle_built):
+    """spectra=1: the E*-{c,fl,fu}.NNNN files of every velocity snapshot, equal to the
+    spectra of the vel-* files written beside them (the reference tool's input)"""
+    from test_gpu_spectra import reference_spectra
+    kw = dict(MICRO, **SEEDED, force_mode=1, g_const="2e-5", ITER=3)
+    odir, gdir, _ = _run_pair(tmp_path, kw, "p")
+    r = subprocess.run([DRIVER, "-p"] + _args(kw) + ["results_dat=0", "spectra=1"], cwd=gdir, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    from helpers import dnsg
+    d = dnsg.dims_of(dnsg.make_params(**MICRO))
+    shapes = {"c": (d.Xc, d.Yc, d.zdc), "fl": (d.Xf, d.Yf, d.zdf), "fu": (d.Xf, d.Yf, d.zdf)}
+    velname = {"c": "vel-c", "fl": "vel-slf", "fu": "vel-suf"}
+    for idx in (0, 3):
+        for tag, (X, Y, Z) in shapes.items():
+            vel = np.fromfile(gdir / f"{velname[tag]}.{idx:04d}").reshape(X, Y, Z, 3)
+            rx, ry = reference_spectra(vel)
+            for c, name in enumerate(("uu", "vv", "ww")):
+                ex = np.fromfile(gdir / f"E{name}x-{tag}.{idx:04d}").reshape(X, Z)
+                ey = np.fromfile(gdir / f"E{name}y-{tag}.{idx:04d}").reshape(Y // 2 + 1, Z)
+                assert np.max(np.abs(ex - rx[c])) <= 1e-12 * max(np.max(np.abs(rx)), 1e-300), (tag, name)
+                assert np.max(np.abs(ey - ry[c])) <= 1e-12 * max(np.max(np.abs(ry)), 1e-300), (tag, name)
+
+
 def test_driver_rejects_unknown_key(tmp_path):
     if not os.path.exists(DRIVER):
         pytest.skip("driver not built")
