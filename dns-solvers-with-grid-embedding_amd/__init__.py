@@ -38,7 +38,7 @@ class Params(ctypes.Structure):
         ("pattern", ctypes.c_int), ("ITER", ctypes.c_long),
         ("force_mode", ctypes.c_int), ("g_const", ctypes.c_double), ("tau_c", ctypes.c_double),
         ("precision", ctypes.c_int), ("init_kind", ctypes.c_int), ("seed", ctypes.c_uint64),
-        ("u_max", ctypes.c_double), ("noise", ctypes.c_double),
+        ("u_max", ctypes.c_double), ("noise", ctypes.c_double), ("fltr", ctypes.c_int),
     ]
 
 

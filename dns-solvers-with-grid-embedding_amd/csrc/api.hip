@@ -82,6 +82,11 @@ int derive(const dnsg_params *p, dnsg_dims *d, std::string *err) {
     if (p->pattern < 0 || p->pattern > 3) return fail("unknown pattern");
     if (p->precision < 0 || p->precision > 2) return fail("unknown precision");
     if (p->force_mode < 0 || p->force_mode > 1) return fail("unknown force mode");
+    if (p->fltr < 0 || p->fltr > 1) return fail("fltr must be 0 (NOFLTR) or 1 (FLTR)");
+    // ftcdtransfer_lesfltr.c:22-28: the filter holds 5 fine layers in an array of GR+1 and
+    // its weights sum to GR^3 only for GR = 4; its slab spans k = zdf-1-GR-2 .. GR+2
+    if (p->fltr && d->GR != 4) return fail("FLTR requires gratio = 4");
+    if (p->fltr && d->zdf - 3 - d->GR < 0) return fail("FLTR requires zds >= gratio + 2");
     return DNSG_OK;
 }
 
@@ -462,7 +467,7 @@ int dnsg_destroy(dnsg_t *h) {
         if (h->lev_up[l]) (void)hipFree(h->lev_up[l]);
     }
     void *ptrs[] = {h->wall_lo, h->wall_up, h->part, h->red, h->seg, h->dsum, h->fsum, h->enc_next,
-                    h->d_force, h->d_diag, h->d_red, h->d_acc};
+                    h->d_force, h->d_diag, h->d_red, h->d_acc, h->fwall, h->fneq};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (h->h_pinned) (void)hipHostFree(h->h_pinned);
@@ -548,11 +553,26 @@ int dnsg::create_ctx(const dnsg_params *p, hipStream_t ext, dnsg_t **out) {
         h->err = "host allocation failed";
         return DNSG_ERR_OOM;
     }
+    if (p->fltr) {
+        HIPCHK(h, hipMalloc(&h->fneq, fltr_fneq_doubles(h) * 8));
+        HIPCHK(h, hipMalloc(&h->fwall, fltr_wall_doubles(h) * 8));
+        HIPCHK(h, hipMemsetAsync(h->fwall, 0, fltr_wall_doubles(h) * 8, h->stream));
+    }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return DNSG_OK;
 }
 
 extern "C" {
+
+// the filtered injection reads the band wall layers on the device
+static int sync_fwall(dnsg_t *h) {
+    if (!h->fwall) return DNSG_OK;
+    const size_t nw = (size_t)h->d.Xf * h->d.Yf * 19;
+    HIPCHK(h, hipMemcpyAsync(h->fwall, h->host_wall_lo, nw * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->fwall + nw, h->host_wall_up, nw * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return DNSG_OK;
+}
 
 static int set_force(dnsg_t *h, double g) {
     h->force = g;
@@ -602,6 +622,8 @@ int dnsg_init_state(dnsg_t *h) {
     }
     h->ss = 0;
     h->have_state = true;
+    int rc = sync_fwall(h);
+    if (rc) return rc;
     return set_force(h, h->p.force_mode == DNSG_FORCE_CONST ? h->p.g_const : 0.);
 }
 
@@ -633,6 +655,7 @@ int dnsg_upload_state(dnsg_t *h, const double *coarse, const double *lower, cons
     if (uin3 && (rc = put_levels(h, uin3, h->lev_up))) return rc;
     h->ss = 0;
     h->have_state = true;
+    if ((rc = sync_fwall(h))) return rc;
     return set_force(h, h->p.force_mode == DNSG_FORCE_CONST ? h->p.g_const : force);
 }
 

@@ -148,6 +148,10 @@ struct dnsg_ctx {
     // mass contribution, for checkpoints and the mass check
     double *host_wall_lo = nullptr, *host_wall_up = nullptr;  // [Xf][Yf][19]
     double wall_mass = 0.;
+    // filtered injection only (p.fltr): device copies of the two wall layers
+    // [lower | upper][Xf][Yf][19], read when the filter's 5 layers reach a wall, and
+    // f_neq of the 5 layers around each coincident layer [side][19][5][Yf][Xf]
+    double *fwall = nullptr, *fneq = nullptr;
 
     double force = 0.;  // host mirror of V.force[0]
     long ss = 0;        // next step index of the time loop
@@ -175,12 +179,17 @@ hipError_t launch_soa_to_aos(hipStream_t s, const void *soa, double *aos, int nx
                              int nz, int prec);
 hipError_t launch_sample(hipStream_t s, dnsg_ctx *h, const void *coarse_state, void *lo, void *up);
 hipError_t launch_interface(hipStream_t s, dnsg_ctx *h, int iter);
+// plain injection (ftcdtransfer.c) or, for a handle with p.fltr, the filtered one
+// (ftcdtransfer_lesfltr.c); both write the same psum layout
 hipError_t launch_ftcd(hipStream_t s, dnsg_ctx *h, double *psum = nullptr);
 // the same kernels over several solves of one precision (one launch per
 // BATCH_MAX items); the interface batch needs one gratio
 hipError_t launch_sample_many(hipStream_t s, dnsg_ctx *const *hs, int n);
 hipError_t launch_interface_many(hipStream_t s, dnsg_ctx *const *hs, int n, int iter);
 hipError_t launch_ftcd_many(hipStream_t s, dnsg_ctx *const *hs, int n, double *psum = nullptr);
+// doubles of the filtered injection's f_neq scratch and of the wall-layer copies
+size_t fltr_fneq_doubles(const dnsg_ctx *h);
+size_t fltr_wall_doubles(const dnsg_ctx *h);
 // workgroups of one injection launch (2 sides per handle), for the psum size
 int ftcd_blocks(const dnsg_ctx *h);
 // bulk momentum and density of each solve from its fused collide-stream sums:

@@ -533,11 +533,180 @@ __global__ __launch_bounds__(256) void k_ftcd(Batch<FtcdItem> B) {
     }
 }
 
+// ---- filtered fine -> coarse injection (ftcdtransfer_lesfltr.c:16-381) ------------------
+// f_neq of the fine nodes of the 5 layers kf-2 .. kf+2 around each band's coincident
+// layer is box-filtered over the 5 x 5 x 5 nodes centred on every coincident node
+// with the trapezoid weights w(ii) w(jj) w(kk), w = {1/2, 1, 1, 1, 1/2} (:22-28),
+// each term added as (w f_neq) / GR^3 in ii, jj, kk order (:222-236); the coarse
+// node gets feq(rho, m) + GR tau_c/tau_f filtered f_neq, where rho and m are the
+// coincident node's unfiltered zeroth and first moments and m, the raw sum e f, is
+// used as the velocity (:173-176, :207-212).  Two launches: one thread per slab
+// node for f_neq (coalesced along x), one thread per coarse node for the filter.
+struct FltrArgs {
+    const void *band[2];    // lower / upper band post-stream lattice (stored layers)
+    const double *wall[2];  // the wall layers k = 0 (lower) and k = zdf-1 (upper), [Xf][Yf][19]
+    int kf[2];              // fine layer coincident with the coarse end layer
+    int kc[2];              // coarse layer written
+    int zdf, Xc, Yc, GR;
+    double fac;             // GR tau_c / tau_f
+    void *coarse;           // coarse post-stream lattice
+    double *fneq;           // [side][19][5][Yf][Xf]
+    double *psum;           // as FtcdItem::psum
+};
+
+// the 19 values of fine node (i, j, k) of one band in double: f (FP64) or the
+// shifted g = f - w (FP32 / FP16 storage); the band's wall layer is not stored in
+// the lattice (lower: stored layer k-1, upper: stored layer k)
+template <class S>
+__device__ __forceinline__ void fltr_node(const FltrArgs &A, int side, int i, int j, int k, double *f) {
+    const int Xf = A.GR * A.Xc, Yf = A.GR * A.Yc;
+    if (side == 0 ? k == 0 : k == A.zdf - 1) {
+        const double *w = A.wall[side] + ((size_t)i * Yf + j) * Q;
+        unroll<0, Q>([&](auto qq) {
+            constexpr int q = decltype(qq)::value;
+            f[q] = sizeof(S) == 8 ? w[q] : w[q] - WQ[q];
+        });
+    } else {
+        const S *b = static_cast<const S *>(A.band[side]);
+        const size_t row = (size_t)(side == 0 ? k - 1 : k) * Yf + j;
+        unroll<0, Q>([&](auto qq) {
+            constexpr int q = decltype(qq)::value;
+            f[q] = (double)load_s(b + lat_at(q, row, i, Xf));
+        });
+    }
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_fltr_slab(FltrArgs A) {
+    const int Xf = A.GR * A.Xc, Yf = A.GR * A.Yc;
+    const size_t plane = (size_t)Xf * Yf;
+    const size_t id = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (id >= 10 * plane) return;
+    const int i = (int)(id % Xf), j = (int)((id / Xf) % Yf), l = (int)(id / plane);
+    const int side = l / 5, kk = l % 5;
+    double f[Q];
+    fltr_node<S>(A, side, i, j, A.kf[side] + kk - 2, f);
+    double *o = A.fneq + ((size_t)side * Q * 5 + kk) * plane + (size_t)j * Xf + i;
+    if constexpr (sizeof(S) == 8) {  // :160-200, the moments in ftcdtransfer's order
+        const Macro64 m = macro64(f);
+        unroll<0, Q>([&](auto qq) {
+            constexpr int q = decltype(qq)::value;
+            o[(size_t)q * 5 * plane] = f[q] - feq64<q>(m.dens, m.u0, m.u1, m.u2, m.v);
+        });
+    } else {
+        const double drho = zeroth_moment(f), dens = 1. + drho;
+        const double u0 = first_moment<0>(f) / dens, u1 = first_moment<1>(f) / dens, u2 = first_moment<2>(f) / dens;
+        const double v = u0 * u0 + u1 * u1 + u2 * u2;
+        unroll<0, Q>([&](auto qq) {
+            constexpr int q = decltype(qq)::value;
+            const double eu = edot<q>(u0, u1, u2);
+            o[(size_t)q * 5 * plane] = f[q] - WQ[q] * (drho + dens * (3. * eu + 4.5 * eu * eu - 1.5 * v));
+        });
+    }
+}
+
+// workgroups [0, nb) filter the lower band, [nb, 2 nb) the upper band (k_ftcd's layout)
+template <class S>
+__global__ __launch_bounds__(256) void k_fltr(FltrArgs A) {
+    const int Xc = A.Xc, Yc = A.Yc, GR = A.GR, Xf = GR * Xc, Yf = GR * Yc;
+    const int n = Xc * Yc, nb = (n + 255) / 256;
+    const int side = (int)blockIdx.x >= nb ? 1 : 0;
+    const int id = ((int)blockIdx.x - side * nb) * 256 + threadIdx.x;
+    double sm = 0., sd = 0.;
+    if (id < n) {
+        const int ic = id % Xc, jc = id / Xc, i = GR * ic, j = GR * jc;
+        const size_t plane = (size_t)Xf * Yf;
+        const double *src = A.fneq + (size_t)side * Q * 5 * plane;
+        constexpr double wl[5] = {0.5, 1., 1., 1., 0.5};
+        const double den = (double)GR * GR * GR;
+        double acc[Q];
+        unroll<0, Q>([&](auto qq) { acc[decltype(qq)::value] = 0.; });
+        for (int ii = 0; ii < 5; ii++) {
+            const int in = (i + ii - 2 + Xf) % Xf;
+            for (int jj = 0; jj < 5; jj++) {
+                const int jn = (j + jj - 2 + Yf) % Yf;
+                const double *p = src + (size_t)jn * Xf + in;
+#pragma unroll
+                for (int kk = 0; kk < 5; kk++) {
+                    const double wt = wl[ii] * wl[jj] * wl[kk];
+                    unroll<0, Q>([&](auto qq) {
+                        constexpr int q = decltype(qq)::value;
+                        acc[q] += (wt * p[((size_t)q * 5 + kk) * plane]) / den;
+                    });
+                }
+            }
+        }
+        double f[Q];
+        fltr_node<S>(A, side, i, j, A.kf[side], f);
+        S *cs = static_cast<S *>(A.coarse);
+        const size_t drow = (size_t)A.kc[side] * Yc + jc;
+        // rho and the raw first moments of the coincident node (not divided by rho)
+        const double dsum = zeroth_moment(f);
+        const double m0 = first_moment<0>(f), m1 = first_moment<1>(f), m2 = first_moment<2>(f);
+        const double v = m0 * m0 + m1 * m1 + m2 * m2;
+        if constexpr (sizeof(S) == 8) {
+            unroll<0, Q>([&](auto qq) {
+                constexpr int q = decltype(qq)::value;
+                const double out = feq64<q>(dsum, m0, m1, m2, v) + A.fac * acc[q];
+                cs[lat_at(q, drow, ic, Xc)] = out;
+                sd += out;
+                if constexpr (EX[q] > 0) sm += out;
+                if constexpr (EX[q] < 0) sm -= out;
+            });
+        } else {  // shifted: feq - w = w (drho + rho (3 eu + 4.5 eu^2 - 1.5 v))
+            const double drho = dsum, dens = 1. + drho;
+            unroll<0, Q>([&](auto qq) {
+                constexpr int q = decltype(qq)::value;
+                const double eu = edot<q>(m0, m1, m2);
+                const double geq = WQ[q] * (drho + dens * (3. * eu + 4.5 * eu * eu - 1.5 * v));
+                const float out = (float)(geq + A.fac * acc[q]);
+                store_s(cs + lat_at(q, drow, ic, Xc), out);
+                const double fv = read_back<S, q>(out);
+                sd += fv;
+                if constexpr (EX[q] > 0) sm += fv;
+                if constexpr (EX[q] < 0) sm -= fv;
+            });
+        }
+    }
+    if (A.psum) {  // uniform per workgroup
+        const double w = 0.5 * GR * GR * GR;
+        sm *= w;
+        sd *= w;
+        wg_sum2(sm, sd);
+        if (threadIdx.x == 0) { A.psum[2 * blockIdx.x] = sm; A.psum[2 * blockIdx.x + 1] = sd; }
+    }
+}
+
+size_t fltr_fneq_doubles(const dnsg_ctx *h) { return 2 * (size_t)Q * 5 * h->d.Xf * h->d.Yf; }
+size_t fltr_wall_doubles(const dnsg_ctx *h) { return 2 * (size_t)h->d.Xf * h->d.Yf * Q; }
+
+template <class S>
+static hipError_t fltr_launch(hipStream_t s, const FltrArgs &A, int nb) {
+    const size_t nslab = 10 * (size_t)A.GR * A.Xc * A.GR * A.Yc;
+    hipLaunchKernelGGL(k_fltr_slab<S>, dim3((unsigned)((nslab + 255) / 256)), dim3(256), 0, s, A);
+    hipLaunchKernelGGL(k_fltr<S>, dim3(nb), dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+static hipError_t launch_fltr(hipStream_t s, dnsg_ctx *h, double *psum) {
+    const int zdf = h->d.zdf, GR = h->d.GR;
+    // the filter's index ranges: GR = 4 and a slab inside each band, 0 <= kf-2, kf+2 <= zdf-1
+    if (GR != 4 || zdf - 3 - GR < 0 || !h->fneq || !h->fwall) return hipErrorInvalidValue;
+    const size_t nw = (size_t)h->d.Xf * h->d.Yf * Q;
+    FltrArgs A = {{h->lower.n(), h->upper.n()}, {h->fwall, h->fwall + nw}, {zdf - 1 - GR, GR}, {0, h->d.zdc - 1},
+                  zdf, h->d.Xc, h->d.Yc, GR, GR * (h->d.omega_f / h->d.omega_c), h->coarse.n(), h->fneq, psum};
+    const int nb = ftcd_blocks(h);
+    if (h->p.precision == DNSG_PREC_FP64) return fltr_launch<double>(s, A, nb);
+    if (h->p.precision == DNSG_PREC_FP32) return fltr_launch<float>(s, A, nb);
+    return fltr_launch<__half>(s, A, nb);
+}
 
 int ftcd_blocks(const dnsg_ctx *h) { return 2 * ((h->d.Xc * h->d.Yc + 255) / 256); }
 
 hipError_t launch_ftcd_many(hipStream_t s, dnsg_ctx *const *hs, int n, double *psum) {
     if (n < 1) return hipSuccess;
+    for (int i = 0; i < n; i++)  // the filtered injection runs per handle (the queue takes none)
+        if (hs[i]->p.fltr) return n == 1 ? launch_fltr(s, hs[0], psum) : hipErrorInvalidValue;
     std::vector<FtcdItem> v(n);
     for (int i = 0; i < n; i++) {
         const dnsg_ctx *h = hs[i];

@@ -331,6 +331,11 @@ int dnsg_queue_submit(dnsg_queue_t *q, const dnsg_params *p, uint64_t id, double
         delete s;
         return DNSG_ERR_ARG;
     }
+    if (p->fltr) {  // the batched ticks run the plain injection only
+        q->err = "queued solves take the plain injection (fltr = 0)";
+        delete s;
+        return DNSG_ERR_ARG;
+    }
     s->bytes = solve_bytes(p, s->d);
     if (s->bytes > q->budget) {
         q->err = "solve does not fit the queue's device-memory budget";

@@ -6,9 +6,10 @@
  *
  * Parameters come from a definitions.h-format file (the reference's own
  * `#define NAME value` lines are parsed, including the RSLIP/PSLIP/SSLIP and
- * ITER switches) and/or key=value arguments with the dnsg_params field names,
- * which also reach the runtime-only extensions (force_mode, g_const, tau_c,
- * precision, init_kind, seed, u_max, noise, results_dat).  print_params=1 prints
+ * ITER switches and the NOFLTR/FLTR injection choice) and/or key=value
+ * arguments with the dnsg_params field names, which also reach fltr and the
+ * runtime-only extensions (force_mode, g_const, tau_c, precision, init_kind,
+ * seed, u_max, noise, results_dat).  print_params=1 prints
  * the parsed parameters and the derived sizes and exits without GPU work;
  * spectra=1 also writes the 1-D energy spectra of every velocity snapshot (the
  * files of the reference's e-spectra tool, computed on the device).
@@ -61,7 +62,7 @@ static int set_param(dnsg_params *p, const char *key, const char *val) {
 #define I(n) if (!strcmp(key, #n)) { p->n = (int)strtol(val, NULL, 10); return 0; }
     D(Rebs) D(ub) I(zdv) I(ydv) I(xdv) I(TSTR) I(TEND) D(CFL) I(gratio) I(zds)
     I(gyplussize) I(wyplussize) I(gxplussize) I(wxplussize) I(pattern)
-    I(force_mode) D(g_const) D(tau_c) I(precision) I(init_kind) D(u_max) D(noise)
+    I(force_mode) D(g_const) D(tau_c) I(precision) I(init_kind) D(u_max) D(noise) I(fltr)
 #undef D
 #undef I
     if (!strcmp(key, "ITER")) { p->ITER = strtol(val, NULL, 10); return 0; }
@@ -80,6 +81,7 @@ static int read_definitions(dnsg_params *p, const char *path) {
     if (!f) return -1;
     p->pattern = DNSG_PATTERN_NONE;
     p->ITER = 0;
+    p->fltr = 0;
     while (fgets(line, sizeof line, f)) {
         char name[128], val[256];
         const char *s = line;
@@ -90,6 +92,8 @@ static int read_definitions(dnsg_params *p, const char *path) {
         if (!strcmp(name, "RSLIP")) p->pattern = DNSG_PATTERN_RSLIP;
         else if (!strcmp(name, "PSLIP")) p->pattern = DNSG_PATTERN_PSLIP;
         else if (!strcmp(name, "SSLIP")) p->pattern = DNSG_PATTERN_SSLIP;
+        else if (!strcmp(name, "FLTR")) p->fltr = 1;   /* main:1369-1371 */
+        else if (!strcmp(name, "NOFLTR")) continue;     /* main:1366-1368, the default */
         else if (val[0]) set_param(p, name, val);
     }
     fclose(f);
@@ -371,6 +375,7 @@ int main(int argc, char **argv) {
                p.Rebs, p.ub, p.xdv, p.ydv, p.zdv, p.TSTR, p.TEND, p.CFL, p.gratio, p.zds);
         printf("gyplussize=%d\nwyplussize=%d\ngxplussize=%d\nwxplussize=%d\npattern=%d\nITER=%ld\n",
                p.gyplussize, p.wyplussize, p.gxplussize, p.wxplussize, p.pattern, p.ITER);
+        printf("fltr=%d\n", p.fltr);
         printf("force_mode=%d\nprecision=%d\ninit_kind=%d\nNZs=%d\nNZ=%d\nutime=%ld\nup=%ld\ntau_c=%.17g\ntau_f=%.17g\n",
                p.force_mode, p.precision, p.init_kind, d.NZs, d.NZ, d.utime, d.up, d.tau_c, d.tau_f);
         return 0;

@@ -101,6 +101,10 @@ typedef struct dnsg_params {
     uint64_t seed;        /* counter-based PRNG seed for DNSG_INIT_LAMINAR_NOISE     */
     double u_max;         /* centreline velocity of the initial parabola             */
     double noise;         /* noise half-width as a fraction of u_max                 */
+    /* definitions.h:87-90: 0 NOFLTR (ftcdtransfer.c), 1 FLTR (ftcdtransfer_lesfltr.c:
+     * f_neq box-filtered over 5 x 5 x 5 fine nodes before injection; the reference's
+     * filter arrays and weights assume gratio = 4, which is required) */
+    int    fltr;
 } dnsg_params;
 
 /* Derived sizes and scalars (main:139-155, 439-469). */

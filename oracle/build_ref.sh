@@ -5,7 +5,7 @@
 #
 # usage: oracle/build_ref.sh <name> key=value...
 #   keys: Rebs ub xdv ydv zdv TSTR TEND CFL gratio zds gyplussize wyplussize
-#         gxplussize wxplussize pattern(0..3) ITER XDIM YDIM
+#         gxplussize wxplussize pattern(0..3) ITER XDIM YDIM fltr(0 NOFLTR, 1 FLTR)
 # output: oracle/_ref/<name>/run-dns-mgrd
 #
 # The reference is configured by editing #define values in definitions.h
@@ -23,7 +23,7 @@ OUT="$HERE/_ref/$name"
 
 declare -A P=( [Rebs]=7200.000 [ub]=0.0866034 [xdv]=64 [ydv]=32 [zdv]=32 [TSTR]=0 [TEND]=1
                [CFL]=0.1 [gratio]=2 [zds]=8 [gyplussize]=56 [wyplussize]=8 [gxplussize]=4
-               [wxplussize]=4 [pattern]=0 [ITER]=0 [XDIM]=1 [YDIM]=1 )
+               [wxplussize]=4 [pattern]=0 [ITER]=0 [XDIM]=1 [YDIM]=1 [fltr]=0 )
 for kv in "$@"; do P[${kv%%=*}]=${kv#*=}; done
 
 [ -d "$SRC" ] || { echo "reference sources not found at $SRC" >&2; exit 3; }
@@ -46,6 +46,8 @@ case ${P[pattern]} in
   3) sedargs+=(-e 's|^//#define SSLIP|#define SSLIP|') ;;
 esac
 if [ "${P[ITER]}" != 0 ]; then sedargs+=(-e "s|^//#define ITER .*|#define ITER ${P[ITER]}|"); fi
+# fine->coarse transfer switch (definitions.h:87-90)
+if [ "${P[fltr]}" = 1 ]; then sedargs+=(-e 's|^#define NOFLTR|//#define NOFLTR|' -e 's|^//#define FLTR$|#define FLTR|'); fi
 sed "${sedargs[@]}" "$SRC/definitions.h" > "$TMP/definitions.h"
 
 objs=()

@@ -687,7 +687,75 @@ static void ftcd_side(orc_t *o, const double *fsrc, int kf, int kc) {
     }
 }
 
+/* ---- filtered injection: ftcdtransfer_lesfltr.c:16-381 (FLTR) --------------------------
+ * f_neq of the 5 fine layers kf-2 .. kf+2 around the coincident layer (:160-200), box-
+ * filtered over 5 x 5 x 5 nodes with the trapezoid weights of :22-28 and summed term by
+ * term as (w f_neq) / GR^3 in ii, jj, kk order (:222-236); rho and the velocity of the
+ * equilibrium come unfiltered from the coincident node, where the reference stores the
+ * raw first moments sum e f (not divided by rho) and uses them as the velocity
+ * (:173-176, :207-212).  Only the post-stream coarse array is written (:242). */
+static void ftcd_flt_side(orc_t *o, const double *fsrc, int kf, int kc) {
+    const int GR = o->GR, Xf = o->Xf, Yf = o->Yf;
+    const double fac = GR * (o->tauf / o->tauc);
+    const double den = (double)GR * GR * GR;
+    static const double wl[5] = {0.5, 1., 1., 1., 0.5}; /* wght[ii][jj][kk] = wl[ii] wl[jj] wl[kk] */
+    double *slab = malloc(sizeof(double) * (size_t)Xf * Yf * 5 * 23);
+    int i, ic;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < Xf; i++) {
+        int j, kk, a;
+        for (j = 0; j < Yf; j++)
+            for (kk = -2; kk < 3; kk++) {
+                const double *fp = fsrc + FI(o, i, j, kf + kk);
+                double *s = slab + (((size_t)i * Yf + j) * 5 + (kk + 2)) * 23;
+                double ux = 0., uy = 0., uz = 0., dens = 0., ueq[3], v;
+                for (a = 0; a < 19; a++) {
+                    ux += E0[a] * fp[a];
+                    uy += E1[a] * fp[a];
+                    uz += E2[a] * fp[a];
+                    dens += fp[a];
+                }
+                ueq[0] = ux / dens;
+                ueq[1] = uy / dens;
+                ueq[2] = uz / dens;
+                v = ueq[0] * ueq[0] + ueq[1] * ueq[1] + ueq[2] * ueq[2];
+                s[0] = dens;
+                s[1] = ux;
+                s[2] = uy;
+                s[3] = uz;
+                for (a = 0; a < 19; a++) s[4 + a] = fp[a] - feq_of(a, dens, ueq, v);
+            }
+    }
+#pragma omp parallel for schedule(static)
+    for (ic = 0; ic < o->Xc; ic++) {
+        int jc, a, ii, jj, kk;
+        for (jc = 0; jc < o->Yc; jc++) {
+            const int i = ic * GR, j = jc * GR;
+            const double *c = slab + (((size_t)i * Yf + j) * 5 + 2) * 23;
+            const double dens = c[0], ueq[3] = {c[1], c[2], c[3]};
+            const double v = ueq[0] * ueq[0] + ueq[1] * ueq[1] + ueq[2] * ueq[2];
+            double fneqt[19], *cf = o->cf + CI(o, ic, jc, kc);
+            for (a = 0; a < 19; a++) fneqt[a] = 0.;
+            for (ii = -2; ii < 3; ii++)
+                for (jj = -2; jj < 3; jj++)
+                    for (kk = -2; kk < 3; kk++) {
+                        const int in = (i + ii + Xf) % Xf, jn = (j + jj + Yf) % Yf;
+                        const double *s = slab + (((size_t)in * Yf + jn) * 5 + (kk + 2)) * 23;
+                        const double wt = wl[ii + 2] * wl[jj + 2] * wl[kk + 2];
+                        for (a = 0; a < 19; a++) fneqt[a] += (wt * s[4 + a]) / den;
+                    }
+            for (a = 0; a < 19; a++) cf[a] = feq_of(a, dens, ueq, v) + fac * fneqt[a];
+        }
+    }
+    free(slab);
+}
+
 void orc_ftcdtransfer(orc_t *o) {
+    if (o->p.fltr) {  /* main:1369-1371 */
+        ftcd_flt_side(o, o->lf, o->zdf - 1 - o->GR, 0);
+        ftcd_flt_side(o, o->uf, o->GR, o->zdc - 1);
+        return;
+    }
     ftcd_side(o, o->lf, o->zdf - 1 - o->GR, 0);     /* :34, :66-67 */
     ftcd_side(o, o->uf, o->GR, o->zdc - 1);         /* :79, :111-112 */
 }

@@ -35,7 +35,7 @@ static int set_param(dnsg_params *p, const char *kv, int *threads) {
 #define L(name) if (!strcmp(key, #name)) { p->name = strtol(val, NULL, 10); return 0; }
     D(Rebs) D(ub) I(zdv) I(ydv) I(xdv) I(TSTR) I(TEND) D(CFL) I(gratio) I(zds)
     I(gyplussize) I(wyplussize) I(gxplussize) I(wxplussize) I(pattern) L(ITER)
-    I(force_mode) D(g_const) D(tau_c) I(precision) I(init_kind) D(u_max) D(noise)
+    I(force_mode) D(g_const) D(tau_c) I(precision) I(init_kind) D(u_max) D(noise) I(fltr)
 #undef D
 #undef I
 #undef L

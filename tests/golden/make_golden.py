@@ -41,6 +41,9 @@ CASES = {
     "micro_gr3": dict(MICRO, gratio=3, zds=5, CFL="5.0", pattern=1, gyplussize=6, wyplussize=6),
     # the reference's own '-s' start: rest state, impulsive flux-controlled start
     "micro_rest": dict(MICRO, pattern=1, gyplussize=4, wyplussize=4, mode="s"),
+    # FLTR: box-filtered fine->coarse injection (ftcdtransfer_lesfltr.c), grid ratio 4;
+    # the 5-layer filter slab reaches both wall layers
+    "micro_fltr": dict(MICRO, gratio=4, zds=6, pattern=1, gyplussize=4, wyplussize=4, fltr=1),
     # the 64x32x32 channel with factor-2 bands of 8 layers, 200 steps
     "cfg1_200": dict(xdv=64, ydv=32, zdv=32, gratio=2, zds=8, Rebs="182.5",
                      ub="0.0333333333333", CFL="4.0", ITER=200),
@@ -48,7 +51,7 @@ CASES = {
 INIT = dict(seed=42, u_max="0.05", noise="0.05")
 LOGS = ["flowrate.txt", "density.txt", "str-up.txt", "str-do.txt", "w-str.txt", "p-grad.txt"]
 REF_KEYS = ["Rebs", "ub", "xdv", "ydv", "zdv", "TSTR", "TEND", "CFL", "gratio", "zds",
-            "gyplussize", "wyplussize", "gxplussize", "wxplussize", "pattern", "ITER"]
+            "gyplussize", "wyplussize", "gxplussize", "wxplussize", "pattern", "ITER", "fltr"]
 
 
 def sha(path):

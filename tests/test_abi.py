@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_params_struct_matches_header_layout():
     # field order/size of the ctypes mirror must match the C struct
-    assert ctypes.sizeof(dnsg.Params) == 144
+    assert ctypes.sizeof(dnsg.Params) == 152
     assert ctypes.sizeof(dnsg.Diag) == 9 * 8
     assert ctypes.sizeof(dnsg.QueueOpts) == 48
     assert ctypes.sizeof(dnsg.SlipResult) == 72

@@ -99,6 +99,14 @@ def test_driver_flux_mode(tmp_path, oracle_built):
 
 
 @pytest.mark.gpu
+def test_driver_filtered_injection(tmp_path, oracle_built):
+    """FLTR (ftcdtransfer_lesfltr.c) through the driver: gratio 4, the thinnest bands"""
+    kw = dict(MICRO, **SEEDED, gratio=4, zds=6, fltr=1, force_mode=1, g_const="2e-5", ITER=20)
+    odir, gdir, _ = _run_pair(tmp_path, kw, "p")
+    _compare(odir, gdir, exact_state=True, tol=1e-10)
+
+
+@pytest.mark.gpu
 def test_driver_rest_start(tmp_path, oracle_built):
     kw = dict(MICRO, init_kind=0)
     odir, gdir, _ = _run_pair(tmp_path, kw, "s")
@@ -120,6 +128,8 @@ DEFINITIONS = """/* a run configuration in the format of the reference's definit
 #define wyplussize 4
 #define gxplussize 4
 #define wxplussize 4
+#define NOFLTR
+//#define FLTR
 //#define PSLIP
 #define RSLIP
 //#define SSLIP
@@ -143,12 +153,25 @@ def test_driver_parses_definitions(tmp_path):
     got = dict(line.split("=", 1) for line in r.stdout.split())
     assert got["pattern"] == "1" and got["ITER"] == "50" and got["xdv"] == "16" and got["zds"] == "4"
     assert float(got["ub"]) == UB and float(got["CFL"]) == 5.0 and got["force_mode"] == "0"
+    assert got["fltr"] == "0"
     from helpers import Oracle, dnsg
     o = Oracle(dnsg.make_params(**MICRO))
     assert int(got["NZs"]) == o.dims.NZs and int(got["NZ"]) == o.dims.NZ
     assert int(got["utime"]) == o.dims.utime and int(got["up"]) == o.dims.up
     assert float(got["tau_c"]) == o.dims.tau_c and float(got["tau_f"]) == o.dims.tau_f
     o.close()
+    # the FLTR switch selects the filtered injection, which needs gratio 4
+    fl = DEFINITIONS.replace("//#define FLTR", "#define FLTR").replace("#define NOFLTR", "//#define NOFLTR")
+    (tmp_path / "definitions.h").write_text(fl.replace("#define gratio 2", "#define gratio 4")
+                                            .replace("#define zds 4", "#define zds 6"))
+    r = subprocess.run([DRIVER, "-s", str(tmp_path / "definitions.h"), "print_params=1"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=30)
+    assert r.returncode == 0, r.stderr
+    assert dict(line.split("=", 1) for line in r.stdout.split())["fltr"] == "1"
+    (tmp_path / "definitions.h").write_text(fl)
+    r = subprocess.run([DRIVER, "-s", str(tmp_path / "definitions.h"), "print_params=1"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=30)
+    assert r.returncode != 0  # FLTR with gratio 2 is rejected
 
 
 @pytest.mark.gpu

@@ -34,6 +34,11 @@ CASES = {
                        pattern="pslip", gxplussize=8, wxplussize=8, gyplussize=4, wyplussize=4),
     "wide_sslip": dict(xdv=256, ydv=6, zdv=9, gratio=2, zds=4, Rebs=20.0, ub=0.0333333333333, CFL=2.0,
                        pattern="sslip", gxplussize=3, wxplussize=5),
+    # filtered injection (FLTR): the thinnest bands, whose filter layers reach the
+    # wall layers, and deeper bands with posts
+    "fltr": dict(MICRO, gratio=4, zds=6, pattern="rslip", gyplussize=4, wyplussize=4, fltr=1),
+    "fltr_deep": dict(xdv=12, ydv=8, zdv=14, gratio=4, zds=8, Rebs=20.0, ub=0.0333333333333, CFL=5.0,
+                      pattern="pslip", gxplussize=8, wxplussize=8, gyplussize=4, wyplussize=4, fltr=1),
 }
 INIT = dict(init_kind="laminar_noise", seed=42, u_max=0.05, noise=0.05)
 GRIDS = ("coarse", "lower", "upper")
@@ -114,12 +119,13 @@ def test_cfg1_200_steps_tier_a():
     assert relmax(dg["flowrate"], dr["flowrate"]) <= 1e-12
 
 
+@pytest.mark.parametrize("name", ["rslip", "fltr"])
 @pytest.mark.parametrize("mode", ["const", "flux"])
-def test_granular_entry_points_equal_step(mode):
+def test_granular_entry_points_equal_step(mode, name):
     """the one-to-one call sequence of main:1256-1375 gives the fused step's state:
     bit for bit under constant force; under the flux controller the fused step takes
     <rho u> from the previous step's writes, a sum in another order (1e-13)"""
-    kw = CASES["rslip"]
+    kw = CASES[name]
     extra = dict(force_mode="const", g_const=2e-5) if mode == "const" else {}
     _, a = pair(kw, **extra)
     _, b = pair(kw, **extra)
@@ -146,7 +152,7 @@ def test_granular_entry_points_equal_step(mode):
             assert relmax(ga[k], gb[k]) <= 1e-13, (k, relmax(ga[k], gb[k]))
 
 
-@pytest.mark.parametrize("name", ["rslip", "wide_pslip", "gr3"])
+@pytest.mark.parametrize("name", ["rslip", "wide_pslip", "gr3", "fltr"])
 def test_flux_controller_from_writes(name):
     """G = ub - <rho u> summed from the previous step's writes (dnsg_step) against the
     same sum taken over the lattices before every step (state reloaded between steps)"""

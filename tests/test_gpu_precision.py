@@ -35,6 +35,23 @@ def test_shifted_tier_against_fp64_oracle(prec):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp16"])
+def test_shifted_tier_filtered_injection(prec):
+    """the same bar with gratio 4 and the filtered injection (FLTR), whose thinnest
+    bands filter over the wall layers"""
+    kw = dict(KW, gratio=4, zds=6, fltr=1)
+    n = 200
+    o = Oracle(dnsg.make_params(**kw), threads=4)
+    o.init_seeded()
+    dr = o.run(n)
+    s = dnsg.Solver(dnsg.make_params(**kw, precision=prec))
+    s.init_state()
+    dg = s.step(n)
+    sg, so = s.statistics(), o.statistics()
+    assert relmax(sg[0], so[0]) <= TOL[prec], relmax(sg[0], so[0])
+    assert relmax(dg["flowrate"], dr["flowrate"]) <= TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
 def test_shifted_state_roundtrip(prec):
     """upload of an FP64 state into shifted storage and back loses only the storage rounding"""
     o = Oracle(dnsg.make_params(**KW))

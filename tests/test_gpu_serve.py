@@ -129,6 +129,14 @@ def test_admission_respects_budget():
     with pytest.raises(dnsg.DnsgError):
         q2 = dnsg.Queue(hbm_budget=per // 2)
         q2.submit(reqs[0], 0)
+    # the batched ticks run the plain injection: a filtered-injection solve is refused
+    filtered = _req(gratio=4)
+    filtered.fltr = 1
+    dnsg.dims_of(filtered)  # valid parameters
+    q3 = dnsg.Queue(max_active=1)
+    with pytest.raises(dnsg.DnsgError, match="fltr"):
+        q3.submit(filtered, 0)
+    q3.close()
 
 
 def test_fp32_slip_matches_fp64():
