@@ -91,6 +91,7 @@ SIGNATURES = {
     "dnsg_get_dims": [_P, ctypes.POINTER(Dims)],
     "dnsg_init_state": [_P],
     "dnsg_upload_state": [_P, _DP, _DP, _DP, _DP, _DP, ctypes.c_double],
+    "dnsg_init_uniform": [_P, _DP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "dnsg_download_state": [_P, _DP, _DP, _DP, _DP, _DP, _DP],
     "dnsg_step": [_P, ctypes.c_int, ctypes.POINTER(Diag)],
     "dnsg_run_timed": [_P, ctypes.c_int, ctypes.POINTER(Diag), _DP],
@@ -229,6 +230,13 @@ class Solver:
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in
                 (coarse, lower, upper, lin, uin)]
         self._check(self.lib.dnsg_upload_state(self.h, *[_ptr(a) for a in arrs], float(force)))
+
+    def init_uniform(self, uniform, xdu, ydu, zdu):
+        """'-i' start from the ftemp half of a uniform-grid lbf file, [xdu][ydu][zdu+2][19]"""
+        u = np.ascontiguousarray(uniform, dtype=np.float64)
+        if u.size != xdu * ydu * (zdu + 2) * 19:
+            raise ValueError("uniform field size does not match xdu, ydu, zdu")
+        self._check(self.lib.dnsg_init_uniform(self.h, _ptr(u), xdu, ydu, zdu))
 
     def download(self):
         s = self.shapes()

@@ -659,6 +659,67 @@ int dnsg_upload_state(dnsg_t *h, const double *coarse, const double *lower, cons
     return set_force(h, h->p.force_mode == DNSG_FORCE_CONST ? h->p.g_const : force);
 }
 
+int dnsg_init_uniform(dnsg_t *h, const double *uniform, int xdu, int ydu, int zdu) {
+    if (!h || !h->stream || !uniform) return DNSG_ERR_ARG;
+    int rc;
+    if ((rc = uniform_check(h, xdu, ydu, zdu, &h->err))) return rc;
+    h->flux_fused = false;
+    const size_t nu = (size_t)xdu * ydu * (zdu + 2) * 19;
+    double *du = nullptr;
+    HIPCHK(h, hipMalloc(&du, nu * 8));
+    hipError_t e = hipMemcpy(du, uniform, nu * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_uniform(h->stream, h, du, xdu, ydu, zdu);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(du);
+    HIPCHK(h, e);
+    // main:986-987 call init() before solid_init(), so the band wall layers start at
+    // rest like every other node (init.c:17-52) and the interpolation leaves them so
+    static const double wq[19] = {1. / 3., 1. / 18., 1. / 18., 1. / 18., 1. / 18., 1. / 18., 1. / 18.,
+                                  1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36.,
+                                  1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36.};
+    const size_t nw = (size_t)h->d.Xf * h->d.Yf;
+    double m = 0.;
+    for (size_t n = 0; n < nw; n++) {
+        double r = 0.;
+        for (int a = 0; a < 19; a++) {
+            h->host_wall_lo[n * 19 + a] = wq[a];
+            h->host_wall_up[n * 19 + a] = wq[a];
+            r += wq[a];
+        }
+        m += r;
+    }
+    h->wall_mass = 2. * m;
+    if ((rc = sync_fwall(h))) return rc;
+    // inter_init.c:150-257: the k = 1 plane's f_neq and the k = zdc-2 plane's f_neq
+    // fill the newest lower / upper levels, but rho and u of both planes are written
+    // into the lower one, so it ends with the upper plane's and the upper level keeps
+    // zero macros; the older levels are copied over the ghosted (Xc+6) x (Yc+6) planes
+    // with a stride of 19 where they have 23, which copies the first 19/23 of each
+    // plane's memory and leaves the rest zero
+    HIPCHK(h, launch_sample(h->stream, h, h->coarse.c(), h->lev_lo[0], h->lev_up[0]));
+    const int Xc = h->d.Xc, Yc = h->d.Yc;
+    const size_t np = (size_t)Xc * Yc * NVP;
+    std::vector<double> lo(3 * np), up(3 * np);
+    if ((rc = get_levels(h, lo.data(), h->lev_lo)) || (rc = get_levels(h, up.data(), h->lev_up))) return rc;
+    const size_t lim = (size_t)19 * (Xc + 6) * (Yc + 6);
+    for (int i = 0; i < Xc; i++)
+        for (int j = 0; j < Yc; j++)
+            for (int v = 0; v < NVP; v++) {
+                const size_t at = ((size_t)i * Yc + j) * NVP + v;
+                if (v < 4) {
+                    lo[at] = up[at];
+                    up[at] = 0.;
+                }
+                const bool copied = ((size_t)(i + 2) * (Yc + 6) + (size_t)(j + 2)) * NVP + v < lim;
+                lo[np + at] = lo[2 * np + at] = copied ? lo[at] : 0.;
+                up[np + at] = up[2 * np + at] = copied ? up[at] : 0.;
+            }
+    if ((rc = put_levels(h, lo.data(), h->lev_lo)) || (rc = put_levels(h, up.data(), h->lev_up))) return rc;
+    h->ss = 0;
+    h->have_state = true;
+    return set_force(h, h->p.force_mode == DNSG_FORCE_CONST ? h->p.g_const : 0.);
+}
+
 int dnsg_download_state(dnsg_t *h, double *coarse, double *lower, double *upper, double *lin3, double *uin3,
                         double *force) {
     if (!h || !h->stream) return DNSG_ERR_ARG;

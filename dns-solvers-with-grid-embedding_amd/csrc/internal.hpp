@@ -187,6 +187,10 @@ hipError_t launch_ftcd(hipStream_t s, dnsg_ctx *h, double *psum = nullptr);
 hipError_t launch_sample_many(hipStream_t s, dnsg_ctx *const *hs, int n);
 hipError_t launch_interface_many(hipStream_t s, dnsg_ctx *const *hs, int n, int iter);
 hipError_t launch_ftcd_many(hipStream_t s, dnsg_ctx *const *hs, int n, double *psum = nullptr);
+// the '-i' start: interpolation of a uniform-grid field [xdu][ydu][zdu+2][19] (device
+// memory) onto the three state lattices, and the check that its stencils stay inside
+int uniform_check(const dnsg_ctx *h, int xdu, int ydu, int zdu, std::string *err);
+hipError_t launch_uniform(hipStream_t s, dnsg_ctx *h, const double *u, int xdu, int ydu, int zdu);
 // doubles of the filtered injection's f_neq scratch and of the wall-layer copies
 size_t fltr_fneq_doubles(const dnsg_ctx *h);
 size_t fltr_wall_doubles(const dnsg_ctx *h);

@@ -14,7 +14,9 @@
  * spectra=1 also writes the 1-D energy spectra of every velocity snapshot (the
  * files of the reference's e-spectra tool, computed on the device).
  *
- * Run modes (main:944-1190): -s starts from rest ('-s' of the reference) or,
+ * Run modes (main:944-1190): -i interpolates the uniform-grid file lbf.TSTR
+ * (sizes xdu, ydu, zdu) onto the grids, as the reference's '-i' does;
+ * -s starts from rest ('-s' of the reference) or,
  * with init_kind=1, from the seeded laminar profile with noise; -p reads the
  * restart files lbf-c/slf/suf/lin/uin.TSTR and force.TSTR from the working
  * directory (their sizes are validated, which the reference does not do).
@@ -42,6 +44,7 @@ static const double E[3][19] = {{0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, 1, -1, -
                                 {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1}};
 
 static int results_dat = 1;
+static int uni_size[3] = {0, 0, 0}; /* xdu, ydu, zdu of the '-i' uniform grid (definitions.h:107-109) */
 static int print_params = 0;
 static int spectra_out = 0;
 
@@ -71,6 +74,9 @@ static int set_param(dnsg_params *p, const char *key, const char *val) {
     if (!strcmp(key, "print_params")) { print_params = atoi(val); return 0; }
     if (!strcmp(key, "spectra")) { spectra_out = atoi(val); return 0; }
     if (!strcmp(key, "XDIM") || !strcmp(key, "YDIM")) return 0; /* one GPU = one rank */
+    if (!strcmp(key, "xdu")) { uni_size[0] = atoi(val); return 0; }
+    if (!strcmp(key, "ydu")) { uni_size[1] = atoi(val); return 0; }
+    if (!strcmp(key, "zdu")) { uni_size[2] = atoi(val); return 0; }
     return -1;
 }
 
@@ -351,7 +357,7 @@ int main(int argc, char **argv) {
     long ss, TM, up, ut;
     double t4, t2, mass;
     if (argc < 2 || argv[1][0] != '-' || !argv[1][1]) {
-        fprintf(stderr, "usage: %s -s|-p [definitions.h] [key=value ...]\n", argv[0]);
+        fprintf(stderr, "usage: %s -s|-p|-i [definitions.h] [key=value ...]\n", argv[0]);
         return 2;
     }
     mode = argv[1][1];
@@ -414,9 +420,19 @@ int main(int argc, char **argv) {
         TM = cntr;
         break;
     }
-    case 'i':
-        fprintf(stderr, "the '-i' uniform-grid initialiser is not part of this build\n");
-        return 2;
+    case 'i': { /* main:982-997: interpolate the uniform-grid file lbf.TSTR */
+        char fn[64];
+        const size_t nu = (size_t)uni_size[0] * uni_size[1] * (uni_size[2] + 2) * 19;
+        double *u = nu ? malloc(nu * 8) : NULL;
+        if (!u) { fprintf(stderr, "'-i' needs the uniform grid sizes xdu, ydu, zdu\n"); return 2; }
+        snprintf(fn, sizeof fn, "lbf.%.4d", p.TSTR); /* initializer.c:303 */
+        fprintf(stderr, "starting to read and interpolate the data 0\n");
+        if (read_doubles(fn, u, nu, 2 * nu)) return 1; /* the ftemp half */
+        if ((rc = dnsg_init_uniform(h, u, uni_size[0], uni_size[1], uni_size[2]))) die(h, "dnsg_init_uniform", rc);
+        free(u);
+        TM = p.TSTR;
+        break;
+    }
     default:
         fprintf(stderr, "not a good file name\n");
         return 1;

@@ -172,6 +172,17 @@ int dnsg_upload_state(dnsg_t *h, const double *coarse, const double *lower,
                       const double *upper, const double *lin3, const double *uin3,
                       double force);
 
+/* '-i' mode (main:982-997): replaces initializer() (initializer.c:15-545, the ftemp
+ * pass, with sp_interpolate_qd.c) and inter_init() (inter_init.c:13-271).  `uniform`
+ * is the ftemp half of the uniform-grid file lbf.NNNN, [xdu][ydu][zdu+2][19]
+ * (definitions.h:107-109 sizes).  Every non-wall node of the three grids gets the
+ * triquadratic interpolant of the uniform field, rescaled to its grid; the wall
+ * layers start at rest; the interface levels follow inter_init, whose indexing
+ * leaves the upper plane's rho and u in the lower level and copies only part of
+ * each plane into the older levels.  The force starts at 0 (g_const in constant-
+ * force mode).  DNSG_ERR_ARG if an interpolation stencil leaves the uniform array. */
+int dnsg_init_uniform(dnsg_t *h, const double *uniform, int xdu, int ydu, int zdu);
+
 /* Checkpoint write path (main:1262-1341): download the current state (the
  * reference's ftemp arrays) in the file layouts above. */
 int dnsg_download_state(dnsg_t *h, double *coarse, double *lower, double *upper,

@@ -1,11 +1,12 @@
 #!/bin/bash
 # Build the reference solver (DNS-solver-with-grid-embedding/src) from its own
 # sources, read in place, for one parameter set.  Used here only, to pin the
-# oracle (tests/golden/make_golden.sh); nothing on the GPU side runs it.
+# oracle (tests/golden/make_golden.py); nothing on the GPU side runs it.
 #
 # usage: oracle/build_ref.sh <name> key=value...
 #   keys: Rebs ub xdv ydv zdv TSTR TEND CFL gratio zds gyplussize wyplussize
 #         gxplussize wxplussize pattern(0..3) ITER XDIM YDIM fltr(0 NOFLTR, 1 FLTR)
+#         xdu ydu zdu (the uniform grid read by the '-i' start)
 # output: oracle/_ref/<name>/run-dns-mgrd
 #
 # The reference is configured by editing #define values in definitions.h
@@ -23,7 +24,8 @@ OUT="$HERE/_ref/$name"
 
 declare -A P=( [Rebs]=7200.000 [ub]=0.0866034 [xdv]=64 [ydv]=32 [zdv]=32 [TSTR]=0 [TEND]=1
                [CFL]=0.1 [gratio]=2 [zds]=8 [gyplussize]=56 [wyplussize]=8 [gxplussize]=4
-               [wxplussize]=4 [pattern]=0 [ITER]=0 [XDIM]=1 [YDIM]=1 [fltr]=0 )
+               [wxplussize]=4 [pattern]=0 [ITER]=0 [XDIM]=1 [YDIM]=1 [fltr]=0
+               [xdu]=512 [ydu]=256 [zdu]=221 )
 for kv in "$@"; do P[${kv%%=*}]=${kv#*=}; done
 
 [ -d "$SRC" ] || { echo "reference sources not found at $SRC" >&2; exit 3; }
@@ -35,7 +37,8 @@ TMP=$(mktemp -d /tmp/dnsg_refbuild.XXXXXX)
 trap 'rm -rf "$TMP"' EXIT
 
 sedargs=()
-for k in Rebs ub zdv ydv xdv TSTR TEND CFL gratio zds gyplussize wyplussize gxplussize wxplussize XDIM YDIM; do
+for k in Rebs ub zdv ydv xdv TSTR TEND CFL gratio zds gyplussize wyplussize gxplussize wxplussize XDIM YDIM \
+         xdu ydu zdu; do
   sedargs+=(-e "s|^#define $k .*|#define $k ${P[$k]}|")
 done
 # surface pattern switches (definitions.h:70-72)

@@ -415,6 +415,254 @@ void orc_init_seeded(orc_t *o) {
     memcpy(o->uinpp, o->uin, (size_t)o->Xc * o->Yc * 23 * 8);
 }
 
+/* ---- '-i' start: initializer.c:15-819 (with sp_interpolate_qd.c:22-45), then
+ * inter_init.c:13-271; main:982-997 ---------------------------------------------
+ * The uniform field is the uniform-grid solver's lbf file, [2][xdu][ydu][zdu+2][19]
+ * (ftemp half, then f half).  With one rank its x/y halo (2 nodes below, 4 above,
+ * filled by the exchange of initializer.c:306-318) holds periodic images, so it is
+ * read with wrapped x/y indices.  Every non-wall node of the three grids gets the
+ * triquadratic Lagrange interpolant of the 3 x 3 x 3 uniform nodes around it,
+ * rescaled f = feq + (dx_u/dx_g)(omega_u/omega_g) f_neq. */
+typedef struct {
+    const double *f;
+    int xu, yu, zu; /* zu = zdu + 2 */
+} uni_t;
+
+static inline int wrapm(int i, int n) {
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+/* sp_interpolate_qd.c:22-45: Lagrange weights on (-1, 0, 1), z then y then x */
+static void interp_qd(const uni_t *u, const int *ind, const int *jnd, const int *knd, double dx, double dy,
+                      double dz, double *ans) {
+    double cz[3][3][19], cy[3][19];
+    int i, j, a;
+    for (i = 0; i < 3; i++)
+        for (j = 0; j < 3; j++) {
+            const size_t col = ((size_t)wrapm(ind[i], u->xu) * u->yu + (size_t)wrapm(jnd[j], u->yu)) * u->zu;
+            const double *f0 = u->f + (col + knd[0]) * 19, *f1 = u->f + (col + knd[1]) * 19,
+                         *f2 = u->f + (col + knd[2]) * 19;
+            for (a = 0; a < 19; a++)
+                cz[i][j][a] = 0.5 * dz * (dz - 1.) * f0[a] - (dz - 1.) * (dz + 1.) * f1[a] + 0.5 * (dz + 1.) * dz * f2[a];
+        }
+    for (i = 0; i < 3; i++)
+        for (a = 0; a < 19; a++)
+            cy[i][a] = 0.5 * dy * (dy - 1.) * cz[i][0][a] - (dy - 1.) * (dy + 1.) * cz[i][1][a] +
+                       0.5 * (dy + 1.) * dy * cz[i][2][a];
+    for (a = 0; a < 19; a++)
+        ans[a] = 0.5 * dx * (dx - 1.) * cy[0][a] - (dx - 1.) * (dx + 1.) * cy[1][a] + 0.5 * (dx + 1.) * dx * cy[2][a];
+}
+
+/* the 3 uniform indices per axis and the offsets (in uniform spacings) of node
+ * (i, j, k) of grid g (0 coarse, 1 lower band, 2 upper band).  half 1 is the second
+ * pass (initializer.c:567-819), which writes two expressions differently from the
+ * first: the lower band's x/y reference index groups (i+0.5)*dxf/dxr otherwise
+ * (:331-332 against :570-571) and the upper band's z offset is formed as
+ * z + dxr/2 - kref dxr (:494 against :731). */
+static void uni_stencil(const orc_t *o, double dxf, double dxc, double dxr, int g, int half, int i, int j, int k,
+                        int *ind, int *jnd, int *knd, double *dx, double *dy, double *dz) {
+    const int GR = o->GR, zdf = o->zdf, zdc = o->zdc;
+    int iref, jref, kref, t;
+    if (g == 1) { /* :326-366, :567-607 */
+        if (half == 0) {
+            iref = (int)floor((i + 0.5) * (dxf / dxr) - 0.5);
+            jref = (int)floor((j + 0.5) * (dxf / dxr) - 0.5);
+        } else {
+            iref = (int)floor((i + 0.5) * dxf / dxr - 0.5);
+            jref = (int)floor((j + 0.5) * dxf / dxr - 0.5);
+        }
+        kref = (int)floor((k - 0.5) * (dxf / dxr) + 0.5);
+        *dx = (i + 0.5) * dxf - (iref + 0.5) * dxr;
+        *dy = (j + 0.5) * dxf - (jref + 0.5) * dxr;
+        if (kref == 0) { /* one-sided at the wall */
+            for (t = 0; t < 3; t++) knd[t] = kref + t;
+            *dz = -dxr + (k - 0.5) * dxf + 0.5 * dxr - kref * dxr;
+        } else {
+            for (t = 0; t < 3; t++) knd[t] = kref - 1 + t;
+            *dz = (k - 0.5) * dxf + 0.5 * dxr - kref * dxr;
+        }
+    } else if (g == 0) { /* :403-431, :642-670 */
+        iref = (int)floor((i + 0.5) * dxc / dxr - 0.5);
+        jref = (int)floor((j + 0.5) * dxc / dxr - 0.5);
+        kref = (int)floor(0.5 + (k * GR + (zdf - GR - 1 - 0.5)) * dxf / dxr);
+        *dx = (i + 0.5) * dxc - (iref + 0.5) * dxr;
+        *dy = (j + 0.5) * dxc - (jref + 0.5) * dxr;
+        for (t = 0; t < 3; t++) knd[t] = kref - 1 + t;
+        *dz = (k * GR + (zdf - GR - 1 - 0.5)) * dxf - (kref - 0.5) * dxr;
+    } else { /* :470-499, :706-734 */
+        iref = (int)floor((i + 0.5) * dxf / dxr - 0.5);
+        jref = (int)floor((j + 0.5) * dxf / dxr - 0.5);
+        kref = (int)floor(0.5 + (k + GR * (zdc - 3) + zdf - 1 - 0.5) * dxf / dxr);
+        *dx = (i + 0.5) * dxf - (iref + 0.5) * dxr;
+        *dy = (j + 0.5) * dxf - (jref + 0.5) * dxr;
+        for (t = 0; t < 3; t++) knd[t] = kref - 1 + t;
+        if (half == 0) *dz = (k + GR * (zdc - 3) + zdf - 1 - 0.5) * dxf - (kref - 0.5) * dxr;
+        else *dz = (k + GR * (zdc - 3) + zdf - 1 - 0.5) * dxf + 0.5 * dxr - kref * dxr; /* :731 */
+    }
+    for (t = 0; t < 3; t++) {
+        ind[t] = iref - 1 + t;
+        jnd[t] = jref - 1 + t;
+    }
+    *dx = *dx / dxr;
+    *dy = *dy / dxr;
+    *dz = *dz / dxr;
+}
+
+/* f = feq + fac (f_int - feq) from the interpolated populations (:367-392) */
+static void uni_rescale(const double *fi, double fac, double *out) {
+    double dens, ueq[3], v;
+    int a;
+    moments(fi, &dens, ueq);
+    v = ueq[0] * ueq[0] + ueq[1] * ueq[1] + ueq[2] * ueq[2];
+    for (a = 0; a < 19; a++) {
+        const double feq = feq_of(a, dens, ueq, v);
+        out[a] = feq + fac * (fi[a] - feq);
+    }
+}
+
+/* the node ranges of the three destination grids: x, y extents and k in [klo, khi) */
+static void uni_ranges(const orc_t *o, int g, int *nx, int *ny, int *klo, int *khi) {
+    *nx = g == 0 ? o->Xc : o->Xf;
+    *ny = g == 0 ? o->Yc : o->Yf;
+    *klo = g == 1 ? 1 : 0;                       /* lower band: k = 1 .. zdf-1      */
+    *khi = g == 0 ? o->zdc : (g == 1 ? o->zdf : o->zdf - 1); /* upper: k = 0 .. zdf-2 */
+}
+
+int orc_init_uniform(orc_t *o, const double *uni, int xdu, int ydu, int zdu, char *err, int errlen) {
+    const int zu = zdu + 2; /* main:140 */
+    const double dxf = 2. / ((double)(o->NZs - 2.)), dxc = o->GR * dxf, dxr = 2. / ((double)(zu - 2.));
+    /* main:443-444: the uniform grid's omega from the same Reynolds number and bulk velocity */
+    const double omega_u = 1. / ((3. / o->p.Rebs) * o->p.ub * (zu - 2.) + 0.5);
+    const size_t nhalf = (size_t)xdu * ydu * zu * 19;
+    int g, half;
+    if (xdu < 1 || ydu < 1 || zdu < 2) {
+        snprintf(err, errlen, "bad uniform grid size");
+        return -1;
+    }
+    /* every stencil must lie inside the uniform array and its halo: x, y in [-2, n+3],
+       z in [0, zdu+1]; the reference reads out of bounds otherwise */
+    for (half = 0; half < 2; half++)
+        for (g = 0; g < 3; g++) {
+            int nx, ny, klo, khi, i, j, k;
+            uni_ranges(o, g, &nx, &ny, &klo, &khi);
+            for (i = 0; i < nx; i++)
+                for (j = 0; j < ny; j++)
+                    for (k = klo; k < khi; k++) {
+                        int ind[3], jnd[3], knd[3];
+                        double dx, dy, dz;
+                        if (j > 0 && i > 0 && j < ny - 1 && i < nx - 1 && k > klo && k < khi - 1) continue;
+                        uni_stencil(o, dxf, dxc, dxr, g, half, i, j, k, ind, jnd, knd, &dx, &dy, &dz);
+                        if (ind[0] < -2 || ind[2] > xdu + 3 || jnd[0] < -2 || jnd[2] > ydu + 3 || knd[0] < 0 ||
+                            knd[2] > zu - 1) {
+                            snprintf(err, errlen, "uniform grid %dx%dx%d does not cover grid %d node (%d,%d,%d)",
+                                     xdu, ydu, zu, g, i, j, k);
+                            return -1;
+                        }
+                    }
+        }
+    /* main:986-987: init() runs before solid_init() here, so no node is marked solid
+       yet and every node, the band wall layers included, starts at rest (init.c:17-52);
+       the interpolation then overwrites every non-wall node.  Levels stay zero. */
+    orc_init_rest(o);
+    {
+        int i, j, a;
+        for (i = 0; i < o->Xf; i++)
+            for (j = 0; j < o->Yf; j++)
+                for (a = 0; a < 19; a++) {
+                    o->lf[FI(o, i, j, 0) + a] = o->lft[FI(o, i, j, 0) + a] = w[a];
+                    o->uf[FI(o, i, j, o->zdf - 1) + a] = o->uft[FI(o, i, j, o->zdf - 1) + a] = w[a];
+                }
+    }
+    for (half = 0; half < 2; half++) {
+        const uni_t u = {uni + half * nhalf, xdu, ydu, zu};
+        double *dst[3];
+        dst[0] = half ? o->cf : o->cft;
+        dst[1] = half ? o->lf : o->lft;
+        dst[2] = half ? o->uf : o->uft;
+        for (g = 0; g < 3; g++) {
+            const double fac = g == 0 ? (dxr / dxc) * (omega_u / o->tauc) : (dxr / dxf) * (omega_u / o->tauf);
+            int nx, ny, klo, khi, i;
+            uni_ranges(o, g, &nx, &ny, &klo, &khi);
+#pragma omp parallel for schedule(static)
+            for (i = 0; i < nx; i++) {
+                int j, k, ind[3], jnd[3], knd[3];
+                double dx, dy, dz, fi[19];
+                for (j = 0; j < ny; j++)
+                    for (k = klo; k < khi; k++) {
+                        double *out = dst[g] + (g == 0 ? CI(o, i, j, k) : FI(o, i, j, k));
+                        uni_stencil(o, dxf, dxc, dxr, g, half, i, j, k, ind, jnd, knd, &dx, &dy, &dz);
+                        interp_qd(&u, ind, jnd, knd, dx, dy, dz, fi);
+                        uni_rescale(fi, fac, out);
+                    }
+            }
+        }
+    }
+    /* inter_init.c:150-205: the k = 1 plane fills lin, the k = zdc-2 plane's f_neq fills
+       uin, but its rho and velocity are written into lin as well (the reference's
+       indexing), so lin carries the upper plane's macros and uin keeps zero macros */
+    {
+        const size_t np = (size_t)o->Xc * o->Yc * 23;
+        const size_t lim = (size_t)19 * (o->Xc + 6) * (o->Yc + 6);
+        double *tl = malloc(np * 8), *tu = malloc(np * 8);
+        size_t n;
+        int i, j, b;
+        if (!tl || !tu) {
+            free(tl); free(tu);
+            snprintf(err, errlen, "out of memory");
+            return -1;
+        }
+        sample_plane(o, o->cft, 1, tl);
+        sample_plane(o, o->cft, o->zdc - 2, tu);
+        for (n = 0; n < np; n += 23) {
+            for (b = 0; b < 4; b++) { o->lin[n + b] = tu[n + b]; o->uin[n + b] = 0.; }
+            for (b = 4; b < 23; b++) { o->lin[n + b] = tl[n + b]; o->uin[n + b] = tu[n + b]; }
+        }
+        /* inter_init.c:245-257: the copy into the older levels runs over the ghosted
+           (Xc+6) x (Yc+6) planes with a stride of 19 where they have 23, so it copies
+           the first 19/23 of each plane's memory and leaves the rest zero */
+        for (i = 0; i < o->Xc; i++)
+            for (j = 0; j < o->Yc; j++)
+                for (b = 0; b < 23; b++) {
+                    const size_t at = PI(o, i, j) + b;
+                    const size_t flat = ((size_t)(i + 2) * (o->Yc + 6) + (size_t)(j + 2)) * 23 + b;
+                    const int copied = flat < lim;
+                    o->linp[at] = o->linpp[at] = copied ? o->lin[at] : 0.;
+                    o->uinp[at] = o->uinpp[at] = copied ? o->uin[at] : 0.;
+                }
+        free(tl);
+        free(tu);
+    }
+    return 0;
+}
+
+/* a synthetic uniform-grid field for the '-i' start, [2][xdu][ydu][zdu+2][19]: each
+ * half is the equilibrium of a laminar profile with seeded noise in rho and u
+ * (different noise in the two halves), wall layers k = 0, zdu+1 at rest */
+void orc_uniform_field(int xdu, int ydu, int zdu, uint64_t seed, double u_max, double noise, double *out) {
+    const int zu = zdu + 2;
+    const size_t nhalf = (size_t)xdu * ydu * zu * 19;
+    int half;
+    for (half = 0; half < 2; half++) {
+        size_t node;
+        for (node = 0; node < nhalf / 19; node++) {
+            const int k = (int)(node % zu);
+            double *f = out + half * nhalf + node * 19;
+            double dens = 1., ueq[3] = {0., 0., 0.}, v;
+            int a;
+            if (k > 0 && k < zu - 1) {
+                const double zeta = (k - 0.5) / (double)(zu - 2), amp = noise * u_max;
+                dens = 1. + 0.01 * noise * orc_prng_sym(seed, 3 + half, 3, node);
+                ueq[0] = 4. * u_max * zeta * (1. - zeta) + amp * orc_prng_sym(seed, 3 + half, 0, node);
+                ueq[1] = amp * orc_prng_sym(seed, 3 + half, 1, node);
+                ueq[2] = amp * orc_prng_sym(seed, 3 + half, 2, node);
+            }
+            v = ueq[0] * ueq[0] + ueq[1] * ueq[1] + ueq[2] * ueq[2];
+            for (a = 0; a < 19; a++) f[a] = feq_of(a, dens, ueq, v);
+        }
+    }
+}
+
 /* ---- interface sampling: xcomputations.c:558-625 ------------------------------ */
 static void sample_plane(const orc_t *o, const double *f, int k, double *plane) {
     int i, j, a;

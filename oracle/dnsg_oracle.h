@@ -20,7 +20,7 @@
  * Pinning (how each function is checked):
  *   - the whole step, masks, logs, statistics, checkpoints: SHA-256 and text
  *     fixtures produced by the reference compiled from its own sources
- *     (oracle/build_ref.sh, tests/golden/make_golden.sh);
+ *     (oracle/build_ref.sh, tests/golden/make_golden.py);
  *   - Hermite, derivatives, bicubic, rescaling, collision invariants, streaming
  *     permutation, wall rules: closed-form tests in tests/test_oracle_math.py.
  */
@@ -62,6 +62,11 @@ void orc_ref_masks(const orc_t *o, int *s_c, int *s_l, int *s_u);
 /* initial states */
 void orc_init_rest(orc_t *o);     /* init.c:15-56 ('-s' mode, walls zero)   */
 void orc_init_seeded(orc_t *o);   /* seeded laminar profile + noise, f=feq  */
+/* '-i' start (main:982-997, initializer.c + inter_init.c) from a uniform-grid lbf
+ * field [2][xdu][ydu][zdu+2][19]; -1 with a message if its stencils leave the
+ * uniform array; and a synthetic such field (seeded laminar profile + noise) */
+int  orc_init_uniform(orc_t *o, const double *uni, int xdu, int ydu, int zdu, char *err, int errlen);
+void orc_uniform_field(int xdu, int ydu, int zdu, uint64_t seed, double u_max, double noise, double *out);
 uint64_t orc_prng_bits(uint64_t seed, int grid, int comp, uint64_t node);
 double   orc_prng_sym(uint64_t seed, int grid, int comp, uint64_t node); /* [-1,1) */
 

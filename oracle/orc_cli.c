@@ -7,6 +7,11 @@
  *                                      reference driver loop (main:1254-1375), writing
  *                                      the same logs, turb-fld and lbf-* files
  *   orc_cli masks <file> key=value...  dump the reference-layout node codes (int32)
+ *   orc_cli uniform <file> key=value... write a synthetic uniform-grid lbf file
+ *                                      (xdu, ydu, zdu, seed, u_max, noise) for '-i'
+ *
+ * init with init_kind=2 is the reference's '-i' start: it reads the uniform-grid
+ * file lbf.TSTR from dir (sizes xdu, ydu, zdu) and interpolates it onto the grids.
  *
  * Keys are the dnsg_params field names (Rebs, ub, xdv, ydv, zdv, gratio, zds, CFL,
  * TSTR, TEND, ITER, pattern, gyplussize, ... , seed, u_max, noise, init_kind,
@@ -18,6 +23,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+
+static int uni_size[3] = {0, 0, 0}; /* xdu, ydu, zdu (definitions.h:107-109) */
 
 static int set_param(dnsg_params *p, const char *kv, int *threads) {
     char key[64];
@@ -41,6 +48,9 @@ static int set_param(dnsg_params *p, const char *kv, int *threads) {
 #undef L
     if (!strcmp(key, "seed")) { p->seed = strtoull(val, NULL, 10); return 0; }
     if (!strcmp(key, "threads")) { *threads = (int)strtol(val, NULL, 10); return 0; }
+    if (!strcmp(key, "xdu")) { uni_size[0] = (int)strtol(val, NULL, 10); return 0; }
+    if (!strcmp(key, "ydu")) { uni_size[1] = (int)strtol(val, NULL, 10); return 0; }
+    if (!strcmp(key, "zdu")) { uni_size[2] = (int)strtol(val, NULL, 10); return 0; }
     return -1;
 }
 
@@ -51,6 +61,26 @@ static void defaults(dnsg_params *p) {
     p->gyplussize = 56; p->wyplussize = 8; p->gxplussize = 4; p->wxplussize = 4;
     p->pattern = DNSG_PATTERN_NONE; p->ITER = 0; p->force_mode = DNSG_FORCE_FLUX;
     p->init_kind = DNSG_INIT_LAMINAR_NOISE; p->seed = 1234; p->u_max = 0.05; p->noise = 0.05;
+}
+
+/* the '-i' start from dir/lbf.TSTR (main:982-997) */
+static int init_uniform(orc_t *o, const char *dir, int tstr) {
+    const size_t n = 2 * (size_t)uni_size[0] * uni_size[1] * (uni_size[2] + 2) * 19;
+    char fn[4096], err[256];
+    double *u;
+    FILE *f;
+    int rc;
+    snprintf(fn, sizeof fn, "%s/lbf.%.4d", dir, tstr); /* initializer.c:303 */
+    if (!n || !(u = malloc(n * 8))) return 2;
+    f = fopen(fn, "rb");
+    rc = !f || fread(u, 8, n, f) != n;
+    if (f) fclose(f);
+    if (!rc && orc_init_uniform(o, u, uni_size[0], uni_size[1], uni_size[2], err, sizeof err)) {
+        fprintf(stderr, "%s\n", err);
+        rc = 1;
+    }
+    free(u);
+    return rc;
 }
 
 int main(int argc, char **argv) {
@@ -69,6 +99,18 @@ int main(int argc, char **argv) {
             return 2;
         }
     if (threads > 0) orc_set_threads(threads);
+    if (!strcmp(argv[1], "uniform")) {
+        const size_t n = 2 * (size_t)uni_size[0] * uni_size[1] * (uni_size[2] + 2) * 19;
+        double *u = n ? malloc(n * 8) : NULL;
+        FILE *f;
+        if (!u) return 2;
+        orc_uniform_field(uni_size[0], uni_size[1], uni_size[2], p.seed, p.u_max, p.noise, u);
+        f = fopen(argv[2], "wb");
+        rc = !f || fwrite(u, 8, n, f) != n;
+        if (f) fclose(f);
+        free(u);
+        return rc ? 1 : 0;
+    }
     o = orc_create(&p, err, sizeof err);
     if (!o) {
         fprintf(stderr, "orc_create: %s\n", err);
@@ -76,9 +118,10 @@ int main(int argc, char **argv) {
     }
     if (!strcmp(argv[1], "init")) {
         if (p.init_kind == DNSG_INIT_REST) orc_init_rest(o);
+        else if (p.init_kind == 2) rc = init_uniform(o, argv[2], p.TSTR);
         else orc_init_seeded(o);
         if (p.force_mode == DNSG_FORCE_FLUX) orc_set_force(o, 0.);
-        rc = orc_write_checkpoint(o, argv[2], p.TSTR);
+        if (!rc) rc = orc_write_checkpoint(o, argv[2], p.TSTR);
     } else if (!strcmp(argv[1], "run")) {
         dnsg_dims d;
         orc_get_dims(o, &d);

@@ -5,7 +5,8 @@ For every case below this script
   1. builds the reference from its own sources for that parameter set
      (oracle/build_ref.sh -> oracle/_ref/<case>/run-dns-mgrd),
   2. writes the seeded initial state as '-p' restart files with the oracle CLI
-     (or uses the reference's own '-s' rest start),
+     (or uses the reference's own '-s' rest start, or, for its '-i' start, writes
+     a synthetic uniform-grid file lbf.0000 with the oracle CLI),
   3. runs the reference on one MPI rank,
   4. stores SHA-256 digests of every checkpoint file it wrote, the six per-step
      text logs, the final turb-fld statistics and 2dresults-lb.bdat in
@@ -44,6 +45,16 @@ CASES = {
     # FLTR: box-filtered fine->coarse injection (ftcdtransfer_lesfltr.c), grid ratio 4;
     # the 5-layer filter slab reaches both wall layers
     "micro_fltr": dict(MICRO, gratio=4, zds=6, pattern=1, gyplussize=4, wyplussize=4, fltr=1),
+    # the '-i' start from a uniform-grid field (initializer.c, inter_init.c): one
+    # uniform grid finer than the fine grid (physically consistent extents) and one
+    # coarser than it, whose first lower-band layer takes the one-sided z stencil.
+    # Only the initializer's output is recorded (the .0000 checkpoint, the step-0 log
+    # lines and the start mass): the step after it reads the older interface levels'
+    # ghost cells, which the reference's level copy leaves unlike their periodic
+    # images (inter_init.c:245-257)
+    "micro_uniform_fine": dict(MICRO, pattern=1, gyplussize=4, wyplussize=4, mode="i", ITER=1,
+                               xdu=64, ydu=32, zdu=50),
+    "micro_uniform_coarse": dict(MICRO, mode="i", ITER=1, xdu=24, ydu=12, zdu=13),
     # the 64x32x32 channel with factor-2 bands of 8 layers, 200 steps
     "cfg1_200": dict(xdv=64, ydv=32, zdv=32, gratio=2, zds=8, Rebs="182.5",
                      ub="0.0333333333333", CFL="4.0", ITER=200),
@@ -51,7 +62,9 @@ CASES = {
 INIT = dict(seed=42, u_max="0.05", noise="0.05")
 LOGS = ["flowrate.txt", "density.txt", "str-up.txt", "str-do.txt", "w-str.txt", "p-grad.txt"]
 REF_KEYS = ["Rebs", "ub", "xdv", "ydv", "zdv", "TSTR", "TEND", "CFL", "gratio", "zds",
-            "gyplussize", "wyplussize", "gxplussize", "wxplussize", "pattern", "ITER", "fltr"]
+            "gyplussize", "wyplussize", "gxplussize", "wxplussize", "pattern", "ITER", "fltr",
+            "xdu", "ydu", "zdu"]
+UNIFORM = dict(seed=77, u_max="0.05", noise="0.1")
 
 
 def sha(path):
@@ -70,12 +83,28 @@ def run_case(name, cfg):
         if mode == "p":
             init_args = ref_args + [f"{k}={v}" for k, v in INIT.items()]
             subprocess.run([ORC_CLI, "init", work] + init_args, check=True)
+        if mode == "i":
+            uni_args = ref_args + [f"{k}={v}" for k, v in UNIFORM.items()]
+            subprocess.run([ORC_CLI, "uniform", os.path.join(work, "lbf.0000")] + uni_args, check=True)
         env = dict(os.environ, LD_LIBRARY_PATH="/opt/conda/lib")
         out = subprocess.run(["/opt/conda/bin/mpiexec", "-n", "1", exe, "-" + mode], cwd=work,
                              env=env, check=True, capture_output=True, text=True).stdout
         fixture = dict(params=cfg, mode=mode, init=INIT if mode == "p" else None, sha256={}, logs={},
                        mass=[l.strip() for l in out.splitlines() if "Average Density" in l])
         files = sorted(os.listdir(work))
+        if mode == "i":
+            fixture["init"] = UNIFORM
+            fixture["mass"] = fixture["mass"][:1]
+            for f in files:
+                if f.endswith(".0000") and f != "lbf.0000" or f.startswith("turb-fld.") and f.endswith(".000"):
+                    fixture["sha256"][f] = sha(os.path.join(work, f))
+            for f in LOGS:
+                with open(os.path.join(work, f)) as fh:
+                    fixture["logs"][f] = fh.readline()
+            with open(os.path.join(HERE, f"{name}.json"), "w") as fh:
+                json.dump(fixture, fh, indent=1, sort_keys=True)
+            print(f"{name}: {len(fixture['sha256'])} digests of the initializer's output")
+            return
         for f in files:
             if f.startswith(("lbf-", "force.")) and not f.endswith(".0000"):
                 fixture["sha256"][f] = sha(os.path.join(work, f))

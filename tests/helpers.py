@@ -61,6 +61,10 @@ def oracle_lib():
         lib.orc_set_levels.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         lib.orc_wall_map.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int8)]
         lib.orc_ref_masks.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 3
+        lib.orc_init_uniform.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        lib.orc_uniform_field.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_double,
+                                          ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         lib.orc_prng_sym.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64]
         lib.orc_prng_sym.restype = ctypes.c_double
         lib.orc_prng_bits.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64]
@@ -136,6 +140,13 @@ class Oracle:
     def init_rest(self):
         self.lib.orc_init_rest(self.o)
 
+    def init_uniform(self, uni, xdu, ydu, zdu):
+        """'-i' start from both halves of a uniform-grid lbf file"""
+        u = np.ascontiguousarray(uni, dtype=np.float64)
+        err = ctypes.create_string_buffer(256)
+        if self.lib.orc_init_uniform(self.o, u.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), xdu, ydu, zdu, err, 256):
+            raise ValueError(err.value.decode())
+
     def step(self, ss):
         d = dnsg.Diag()
         rc = self.lib.orc_step(self.o, ss, ctypes.byref(d))
@@ -164,3 +175,10 @@ def relmax(a, b):
     b = np.asarray(b, dtype=np.float64)
     den = np.max(np.abs(b))
     return float(np.max(np.abs(a - b)) / (den if den > 0 else 1.0))
+
+
+def uniform_field(xdu, ydu, zdu, seed=77, u_max=0.05, noise=0.1):
+    """the oracle's synthetic uniform-grid lbf field, [2][xdu][ydu][zdu+2][19]"""
+    out = np.empty((2, xdu, ydu, zdu + 2, 19))
+    oracle_lib().orc_uniform_field(xdu, ydu, zdu, seed, u_max, noise, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out

@@ -107,6 +107,26 @@ def test_driver_filtered_injection(tmp_path, oracle_built):
 
 
 @pytest.mark.gpu
+def test_driver_uniform_start(tmp_path, oracle_built):
+    """'-i': both programs interpolate the same uniform-grid file lbf.0000 and run on"""
+    kw = dict(MICRO, force_mode=1, g_const="2e-5", ITER=10, xdu=24, ydu=12, zdu=13)
+    odir, gdir = tmp_path / "oracle", tmp_path / "driver"
+    odir.mkdir()
+    gdir.mkdir()
+    subprocess.run([ORC_CLI, "uniform", str(odir / "lbf.0000")] + _args(kw) + ["seed=77", "noise=0.1"], check=True)
+    shutil.copy(odir / "lbf.0000", gdir / "lbf.0000")
+    subprocess.run([ORC_CLI, "init", str(odir)] + _args(kw) + ["init_kind=2"], check=True)
+    subprocess.run([ORC_CLI, "run", str(odir)] + _args(kw) + ["threads=4"], check=True, capture_output=True)
+    print("driver: dnsg-run -i", flush=True)
+    r = subprocess.run([DRIVER, "-i"] + _args(kw) + ["results_dat=0"], cwd=gdir, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    (odir / "lbf.0000").unlink()  # the input, not an output
+    (gdir / "lbf.0000").unlink()
+    _compare(odir, gdir, exact_state=True, tol=1e-10)
+
+
+@pytest.mark.gpu
 def test_driver_rest_start(tmp_path, oracle_built):
     kw = dict(MICRO, init_kind=0)
     odir, gdir, _ = _run_pair(tmp_path, kw, "s")

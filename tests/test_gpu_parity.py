@@ -16,7 +16,7 @@ tests/test_oracle_golden.py.
 import numpy as np
 import pytest
 
-from helpers import Oracle, dnsg, relmax
+from helpers import Oracle, dnsg, relmax, uniform_field
 
 pytestmark = pytest.mark.gpu
 
@@ -52,6 +52,67 @@ def pair(kw, **extra):
     st = o.state()
     s.upload(st["coarse"], st["lower"], st["upper"], st["lin"], st["uin"], st["force"])
     return o, s
+
+
+# '-i' start (initializer.c, inter_init.c): uniform grids finer than the fine grid with
+# consistent extents, and coarser than it (one-sided z stencil at the lower wall)
+UNIFORM = {
+    "uniform_fine": (dict(MICRO, pattern="rslip", gyplussize=4, wyplussize=4), (64, 32, 50)),
+    "uniform_coarse": (dict(MICRO), (24, 12, 13)),
+}
+
+
+@pytest.mark.parametrize("name", list(UNIFORM))
+def test_uniform_start_bitwise(name):
+    """the interpolated lattices, the wall layers and the three interface levels equal the
+    oracle's (itself pinned to the reference's step-0 checkpoint) bit for bit, and stay
+    so for 25 constant-force steps"""
+    kw, (xu, yu, zu) = UNIFORM[name]
+    uni = uniform_field(xu, yu, zu)
+    p = dnsg.make_params(**kw, force_mode="const", g_const=2e-5)
+    o = Oracle(p, threads=4)
+    o.init_uniform(uni, xu, yu, zu)
+    s = dnsg.Solver(p)
+    print(f"uniform start {name}: {xu}x{yu}x{zu + 2}", flush=True)
+    s.init_uniform(uni[0], xu, yu, zu)
+    g, ref = s.download(), o.state()
+    for k in GRIDS + ("lin", "uin"):
+        assert np.array_equal(g[k], ref[k]), f"{k}: max diff {np.max(np.abs(g[k] - ref[k]))}"
+    assert abs(s.mass() - o.mass()) <= 1e-14
+    dg = s.step(25)
+    dr = o.run(25)
+    g, ref = s.download(), o.state()
+    for k in GRIDS + ("lin", "uin"):
+        assert np.array_equal(g[k], ref[k]), f"{k}: max diff {np.max(np.abs(g[k] - ref[k]))}"
+    assert relmax(dg["flowrate"], dr["flowrate"]) <= 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+def test_uniform_start_shifted_tiers(prec):
+    """shifted storage of the same interpolated state: only the storage rounding differs"""
+    kw, (xu, yu, zu) = UNIFORM["uniform_coarse"]
+    uni = uniform_field(xu, yu, zu)
+    o = Oracle(dnsg.make_params(**kw), threads=4)
+    o.init_uniform(uni, xu, yu, zu)
+    s = dnsg.Solver(dnsg.make_params(**kw, precision=prec))
+    s.init_uniform(uni[0], xu, yu, zu)
+    g, ref = s.download(), o.state()
+    eps = {"fp32": 2e-8, "fp16": 2e-4}[prec]  # absolute, on f ~ w +- 1e-2
+    for k in GRIDS:
+        assert np.max(np.abs(g[k] - ref[k])) <= eps, k
+
+
+def test_uniform_start_rejects_uncovered_grid():
+    """a uniform grid whose interpolation stencils would leave the array (the reference
+    reads out of bounds there) is refused"""
+    kw, (xu, yu, zu) = UNIFORM["uniform_coarse"]
+    uni = uniform_field(4, yu, zu)
+    s = dnsg.Solver(dnsg.make_params(**kw))
+    with pytest.raises(dnsg.DnsgError, match="does not cover"):
+        s.init_uniform(uni[0], 4, yu, zu)
+    o = Oracle(dnsg.make_params(**kw))
+    with pytest.raises(ValueError, match="does not cover"):
+        o.init_uniform(uni, 4, yu, zu)
 
 
 @pytest.mark.parametrize("name", list(CASES))
