@@ -7,6 +7,11 @@
  *                                      reference driver loop (main:1254-1375), writing
  *                                      the same logs, turb-fld and lbf-* files
  *   orc_cli masks <file> key=value...  dump the reference-layout node codes (int32)
+ *   orc_cli budget <dir> key=value...  the TKE-budget tool's per-snapshot pass
+ *                                      (budget-no-slip-mgrd.c:82-92): for TSTR..TEND
+ *                                      read <dir>/../vel-*, den-*.NNNN and write
+ *                                      turb-fld-2d, vort-fluc-2d, KE-Budget-2d
+ *                                      .000.NNNN (and ufl4/vfl4 every 200) into dir
  *   orc_cli uniform <file> key=value... write a synthetic uniform-grid lbf file
  *                                      (xdu, ydu, zdu, seed, u_max, noise) for '-i'
  *
@@ -18,6 +23,7 @@
  * force_mode, g_const, tau_c, threads).
  */
 #define _POSIX_C_SOURCE 200809L
+#include "budget_oracle.h"
 #include "dnsg_oracle.h"
 
 #include <stdio.h>
@@ -61,6 +67,62 @@ static void defaults(dnsg_params *p) {
     p->gyplussize = 56; p->wyplussize = 8; p->gxplussize = 4; p->wxplussize = 4;
     p->pattern = DNSG_PATTERN_NONE; p->ITER = 0; p->force_mode = DNSG_FORCE_FLUX;
     p->init_kind = DNSG_INIT_LAMINAR_NOISE; p->seed = 1234; p->u_max = 0.05; p->noise = 0.05;
+}
+
+static int read_all(const char *fn, double *dst, size_t n) {
+    FILE *f = fopen(fn, "rb");
+    int bad = !f || fread(dst, 8, n, f) != n;
+    if (f) fclose(f);
+    if (bad) fprintf(stderr, "cannot read %s\n", fn);
+    return bad;
+}
+
+static int write_all(const char *fn, const double *src, size_t n) {
+    FILE *f = fopen(fn, "wb");
+    int bad = !f || fwrite(src, 8, n, f) != n;
+    if (f) fclose(f);
+    return bad;
+}
+
+/* the budget tool's snapshot loop (budget-no-slip-mgrd.c:82-92) */
+static int budget(const dnsg_params *p, const char *dir) {
+    const int GR = p->gratio, zdf = p->zds + 1, NZ = orc_budget_profile_len(p);
+    const size_t nc = (size_t)p->xdv * p->ydv * p->zdv, nf = (size_t)GR * p->xdv * GR * p->ydv * zdf;
+    double *vc = malloc(nc * 24), *dc = malloc(nc * 8), *vl = malloc(nf * 24), *dl = malloc(nf * 8);
+    double *vu = malloc(nf * 24), *du = malloc(nf * 8), *turb = malloc((size_t)19 * NZ * 8);
+    double *vort = malloc((size_t)3 * NZ * 8), *ke = malloc((size_t)30 * NZ * 8);
+    const size_t npl = (size_t)GR * p->xdv * GR * p->ydv;
+    double *ufl = malloc(npl * 8), *vfl = malloc(npl * 8);
+    char fn[4096];
+    int ts, rc = 0;
+    if (!vc || !dc || !vl || !dl || !vu || !du || !turb || !vort || !ke || !ufl || !vfl) rc = 2;
+    for (ts = p->TSTR; !rc && ts <= p->TEND; ts++) {
+        const int dump = !(ts % 200);
+#define RD(name, buf, n) (snprintf(fn, sizeof fn, "%s/../" name ".%.4d", dir, ts), read_all(fn, buf, n))
+        rc = RD("vel-c", vc, 3 * nc) || RD("vel-slf", vl, 3 * nf) || RD("vel-suf", vu, 3 * nf) ||
+             RD("den-c", dc, nc) || RD("den-slf", dl, nf) || RD("den-suf", du, nf);
+#undef RD
+        if (rc) break;
+        if (orc_budget2d(p, vc, dc, vl, dl, vu, du, turb, vort, ke, dump ? ufl : NULL, dump ? vfl : NULL)) {
+            rc = 2;
+            break;
+        }
+        snprintf(fn, sizeof fn, "%s/turb-fld-2d.%.3d.%.4d", dir, 0, ts);
+        rc |= write_all(fn, turb, (size_t)19 * NZ);
+        snprintf(fn, sizeof fn, "%s/vort-fluc-2d.%.3d.%.4d", dir, 0, ts);
+        rc |= write_all(fn, vort, (size_t)3 * NZ);
+        snprintf(fn, sizeof fn, "%s/KE-Budget-2d.%.3d.%.4d", dir, 0, ts);
+        rc |= write_all(fn, ke, (size_t)30 * NZ);
+        if (dump) {
+            snprintf(fn, sizeof fn, "%s/ufl4.%.4d", dir, ts);
+            rc |= write_all(fn, ufl, npl);
+            snprintf(fn, sizeof fn, "%s/vfl4.%.4d", dir, ts);
+            rc |= write_all(fn, vfl, npl);
+        }
+    }
+    free(vc); free(dc); free(vl); free(dl); free(vu); free(du); free(turb); free(vort); free(ke);
+    free(ufl); free(vfl);
+    return rc;
 }
 
 /* the '-i' start from dir/lbf.TSTR (main:982-997) */
@@ -111,6 +173,7 @@ int main(int argc, char **argv) {
         free(u);
         return rc ? 1 : 0;
     }
+    if (!strcmp(argv[1], "budget")) return budget(&p, argv[2]) ? 1 : 0;
     o = orc_create(&p, err, sizeof err);
     if (!o) {
         fprintf(stderr, "orc_create: %s\n", err);

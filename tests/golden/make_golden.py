@@ -12,6 +12,12 @@ For every case below this script
      text logs, the final turb-fld statistics and 2dresults-lb.bdat in
      tests/golden/<case>.json.
 
+The BUDGET cases pin the TKE-budget oracle (oracle/budget_oracle.c) the same way:
+the oracle's solver writes a few velocity/density snapshots, the reference's
+budget tool (TKE-budget/computations, built by oracle/build_budget_ref.sh) runs on
+them from a subdirectory, as its README prescribes, and the digests of every file
+it writes are stored.
+
 It runs only where the reference sources and MPICH are present; the fixtures
 it writes are plain data (digests, text logs, numbers) and are what the CPU
 tests compare the oracle against.
@@ -28,6 +34,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 ORC_CLI = os.path.join(REPO, "oracle", "build", "orc_cli")
 BUILD_REF = os.path.join(REPO, "oracle", "build_ref.sh")
+BUILD_BUDGET = os.path.join(REPO, "oracle", "build_budget_ref.sh")
 
 MICRO = dict(xdv=16, ydv=8, zdv=12, gratio=2, zds=4, Rebs="20.0", ub="0.0333333333333",
              CFL="5.0", ITER=50)
@@ -60,6 +67,16 @@ CASES = {
                      ub="0.0333333333333", CFL="4.0", ITER=200),
 }
 INIT = dict(seed=42, u_max="0.05", noise="0.05")
+# snapshots TSTR..TEND of a short solver run, then the budget tool over them: the
+# thinnest bands (the one-sided z stencils reach into the neighbouring rows there)
+# with the fluctuation dump of snapshot 0, and deeper bands at grid ratio 3
+BUDGET = {
+    "budget_micro": dict(solver=dict(MICRO, pattern=1, gyplussize=4, wyplussize=4, ITER=3), TSTR=0, TEND=2),
+    "budget_gr3": dict(solver=dict(xdv=12, ydv=8, zdv=14, gratio=3, zds=8, Rebs="20.0", ub="0.0333333333333",
+                                   CFL="5.0", ITER=3, pattern=2, gxplussize=6, wxplussize=6, gyplussize=6,
+                                   wyplussize=6), TSTR=1, TEND=3),
+}
+BUDGET_KEYS = ["Rebs", "ub", "xdv", "ydv", "zdv", "CFL", "gratio", "zds"]
 LOGS = ["flowrate.txt", "density.txt", "str-up.txt", "str-do.txt", "w-str.txt", "p-grad.txt"]
 REF_KEYS = ["Rebs", "ub", "xdv", "ydv", "zdv", "TSTR", "TEND", "CFL", "gratio", "zds",
             "gyplussize", "wyplussize", "gxplussize", "wxplussize", "pattern", "ITER", "fltr",
@@ -123,9 +140,36 @@ def run_case(name, cfg):
         shutil.rmtree(work)
 
 
+def run_budget_case(name, cfg):
+    solver = cfg["solver"]
+    args = [f"{k}={v}" for k, v in solver.items()] + [f"{k}={v}" for k, v in INIT.items()]
+    tool = [f"{k}={solver[k]}" for k in BUDGET_KEYS] + [f"TSTR={cfg['TSTR']}", f"TEND={cfg['TEND']}"]
+    subprocess.run([BUILD_BUDGET, name] + tool, check=True, stdout=subprocess.DEVNULL)
+    exe = os.path.join(REPO, "oracle", "_ref", name, "compute-budget")
+    work = tempfile.mkdtemp(prefix=f"dnsg_golden_{name}_")
+    try:
+        subprocess.run([ORC_CLI, "init", work] + args, check=True)
+        subprocess.run([ORC_CLI, "run", work] + args + ["threads=4"], check=True, capture_output=True)
+        post = os.path.join(work, "post")
+        os.mkdir(post)
+        env = dict(os.environ, LD_LIBRARY_PATH="/opt/conda/lib")
+        subprocess.run(["/opt/conda/bin/mpiexec", "-n", "1", exe], cwd=post, env=env, check=True,
+                       capture_output=True)
+        fixture = dict(kind="budget", params=solver, init=INIT, TSTR=cfg["TSTR"], TEND=cfg["TEND"],
+                       sha256={f: sha(os.path.join(post, f)) for f in sorted(os.listdir(post))})
+        with open(os.path.join(HERE, f"{name}.json"), "w") as fh:
+            json.dump(fixture, fh, indent=1, sort_keys=True)
+        print(f"{name}: {len(fixture['sha256'])} budget-tool files")
+    finally:
+        shutil.rmtree(work)
+
+
 if __name__ == "__main__":
     if not os.path.exists(ORC_CLI):
         subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], check=True)
-    wanted = sys.argv[1:] or list(CASES)
+    wanted = sys.argv[1:] or list(CASES) + list(BUDGET)
     for n in wanted:
-        run_case(n, CASES[n])
+        if n in BUDGET:
+            run_budget_case(n, BUDGET[n])
+        else:
+            run_case(n, CASES[n])

@@ -22,7 +22,8 @@ import pytest
 
 from conftest import GOLDEN, ORC_CLI
 
-CASES = sorted(glob.glob(os.path.join(GOLDEN, "*.json")))
+CASES = sorted(c for c in glob.glob(os.path.join(GOLDEN, "*.json")) if json.load(open(c)).get("kind") != "budget")
+BUDGET_CASES = sorted(c for c in glob.glob(os.path.join(GOLDEN, "*.json")) if json.load(open(c)).get("kind") == "budget")
 
 
 def _sha(path):
@@ -56,3 +57,20 @@ def test_oracle_matches_reference(case, tmp_path, oracle_built):
     got_mass = [re.search(r":\s*([-0-9.eE+]+)", l).group(1) for l in out.splitlines() if "Average" in l]
     got_mass = got_mass[: len(ref_mass)]
     assert got_mass == ref_mass
+
+
+@pytest.mark.parametrize("case", BUDGET_CASES, ids=[os.path.basename(c)[:-5] for c in BUDGET_CASES])
+def test_budget_oracle_matches_reference_tool(case, tmp_path, oracle_built):
+    """the TKE-budget restatement against the reference's budget tool run on the same
+    snapshots (TKE-budget/computations): every file it writes, byte for byte"""
+    fx = json.load(open(case))
+    args = [f"{k}={v}" for k, v in fx["params"].items()] + [f"{k}={v}" for k, v in fx["init"].items()]
+    subprocess.run([ORC_CLI, "init", str(tmp_path)] + args, check=True)
+    subprocess.run([ORC_CLI, "run", str(tmp_path)] + args + ["threads=4"], check=True, capture_output=True)
+    post = tmp_path / "post"
+    post.mkdir()
+    subprocess.run([ORC_CLI, "budget", str(post)] + args + [f"TSTR={fx['TSTR']}", f"TEND={fx['TEND']}"],
+                   check=True)
+    assert sorted(os.listdir(post)) == sorted(fx["sha256"])
+    for name, digest in fx["sha256"].items():
+        assert _sha(post / name) == digest, f"{name} differs from the reference tool's"
