@@ -92,6 +92,7 @@ SIGNATURES = {
     "dnsg_init_state": [_P],
     "dnsg_upload_state": [_P, _DP, _DP, _DP, _DP, _DP, ctypes.c_double],
     "dnsg_init_uniform": [_P, _DP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "dnsg_budget2d": [_P, _DP, _DP, _DP, _DP, _DP],
     "dnsg_download_state": [_P, _DP, _DP, _DP, _DP, _DP, _DP],
     "dnsg_step": [_P, ctypes.c_int, ctypes.POINTER(Diag)],
     "dnsg_run_timed": [_P, ctypes.c_int, ctypes.POINTER(Diag), _DP],
@@ -237,6 +238,16 @@ class Solver:
         if u.size != xdu * ydu * (zdu + 2) * 19:
             raise ValueError("uniform field size does not match xdu, ydu, zdu")
         self._check(self.lib.dnsg_init_uniform(self.h, _ptr(u), xdu, ydu, zdu))
+
+    def budget2d(self, planes=False):
+        """the TKE-budget tool's per-snapshot profiles of the current state:
+        (turb [19][NZ], vort [3][NZ], ke [30][NZ], (ufl, vfl) [Xf][Yf] or None)"""
+        NZ = self.dims.NZ
+        turb, vort, ke = np.empty((19, NZ)), np.empty((3, NZ)), np.empty((30, NZ))
+        uv = (np.empty((self.dims.Xf, self.dims.Yf)), np.empty((self.dims.Xf, self.dims.Yf))) if planes else None
+        self._check(self.lib.dnsg_budget2d(self.h, _ptr(turb), _ptr(vort), _ptr(ke), _ptr(uv[0]) if uv else None,
+                                           _ptr(uv[1]) if uv else None))
+        return turb, vort, ke, uv
 
     def download(self):
         s = self.shapes()

@@ -574,6 +574,15 @@ static int sync_fwall(dnsg_t *h) {
     return DNSG_OK;
 }
 
+}  // extern "C"
+
+int dnsg::ensure_fwall(dnsg_t *h) {
+    if (!h->fwall) HIPCHK(h, hipMalloc(&h->fwall, fltr_wall_doubles(h) * 8));
+    return sync_fwall(h);
+}
+
+extern "C" {
+
 static int set_force(dnsg_t *h, double g) {
     h->force = g;
     HIPCHK(h, hipMemcpyAsync(h->d_force, &h->force, 8, hipMemcpyHostToDevice, h->stream));

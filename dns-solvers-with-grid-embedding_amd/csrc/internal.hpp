@@ -191,6 +191,9 @@ hipError_t launch_ftcd_many(hipStream_t s, dnsg_ctx *const *hs, int n, double *p
 // memory) onto the three state lattices, and the check that its stencils stay inside
 int uniform_check(const dnsg_ctx *h, int xdu, int ydu, int zdu, std::string *err);
 hipError_t launch_uniform(hipStream_t s, dnsg_ctx *h, const double *u, int xdu, int ydu, int zdu);
+// device copies of the band wall layers (fwall), allocated on first use and
+// refreshed from the host copies
+int ensure_fwall(dnsg_t *h);
 // doubles of the filtered injection's f_neq scratch and of the wall-layer copies
 size_t fltr_fneq_doubles(const dnsg_ctx *h);
 size_t fltr_wall_doubles(const dnsg_ctx *h);

@@ -12,7 +12,8 @@
  * seed, u_max, noise, results_dat).  print_params=1 prints
  * the parsed parameters and the derived sizes and exits without GPU work;
  * spectra=1 also writes the 1-D energy spectra of every velocity snapshot (the
- * files of the reference's e-spectra tool, computed on the device).
+ * files of the reference's e-spectra tool, computed on the device); budget=1 the
+ * per-snapshot statistics and energy budget of its TKE-budget tool.
  *
  * Run modes (main:944-1190): -i interpolates the uniform-grid file lbf.TSTR
  * (sizes xdu, ydu, zdu) onto the grids, as the reference's '-i' does;
@@ -47,6 +48,7 @@ static int results_dat = 1;
 static int uni_size[3] = {0, 0, 0}; /* xdu, ydu, zdu of the '-i' uniform grid (definitions.h:107-109) */
 static int print_params = 0;
 static int spectra_out = 0;
+static int budget_out = 0;
 
 static double now(void) {
     struct timespec t;
@@ -73,6 +75,7 @@ static int set_param(dnsg_params *p, const char *key, const char *val) {
     if (!strcmp(key, "results_dat")) { results_dat = atoi(val); return 0; }
     if (!strcmp(key, "print_params")) { print_params = atoi(val); return 0; }
     if (!strcmp(key, "spectra")) { spectra_out = atoi(val); return 0; }
+    if (!strcmp(key, "budget")) { budget_out = atoi(val); return 0; }
     if (!strcmp(key, "XDIM") || !strcmp(key, "YDIM")) return 0; /* one GPU = one rank */
     if (!strcmp(key, "xdu")) { uni_size[0] = atoi(val); return 0; }
     if (!strcmp(key, "ydu")) { uni_size[1] = atoi(val); return 0; }
@@ -196,6 +199,34 @@ static void spectra_print(dnsg_t *h, const dnsg_dims *d, long cntr) {
         }
         free(ex); free(ey);
     }
+}
+
+/* TKE-budget/computations/budget2D.c:1550-1636 and uv_fluc.c:35-45 file part: the
+   tool's per-snapshot files turb-fld-2d, vort-fluc-2d, KE-Budget-2d.000.NNNN and,
+   every 200th snapshot, ufl4/vfl4.NNNN, from the current state */
+static void budget_print(dnsg_t *h, const dnsg_dims *d, long cntr) {
+    const int NZ = d->NZ, dump = !(cntr % 200);
+    const size_t np = (size_t)d->Xf * d->Yf;
+    double *turb = malloc(sizeof(double) * 19 * NZ), *vort = malloc(sizeof(double) * 3 * NZ);
+    double *ke = malloc(sizeof(double) * 30 * NZ);
+    double *ufl = dump ? malloc(sizeof(double) * np) : NULL, *vfl = dump ? malloc(sizeof(double) * np) : NULL;
+    char fn[64];
+    int rc;
+    if (!turb || !vort || !ke || (dump && (!ufl || !vfl))) { fprintf(stderr, "budget: out of memory\n"); exit(1); }
+    if ((rc = dnsg_budget2d(h, turb, vort, ke, ufl, vfl))) die(h, "dnsg_budget2d", rc);
+    snprintf(fn, sizeof fn, "turb-fld-2d.%.3d.%.4d", 0, (int)cntr);
+    write_file(fn, turb, (size_t)19 * NZ, NULL, 0);
+    snprintf(fn, sizeof fn, "vort-fluc-2d.%.3d.%.4d", 0, (int)cntr);
+    write_file(fn, vort, (size_t)3 * NZ, NULL, 0);
+    snprintf(fn, sizeof fn, "KE-Budget-2d.%.3d.%.4d", 0, (int)cntr);
+    write_file(fn, ke, (size_t)30 * NZ, NULL, 0);
+    if (dump) {
+        snprintf(fn, sizeof fn, "ufl4.%.4d", (int)cntr);
+        write_file(fn, ufl, np, NULL, 0);
+        snprintf(fn, sizeof fn, "vfl4.%.4d", (int)cntr);
+        write_file(fn, vfl, np, NULL, 0);
+    }
+    free(turb); free(vort); free(ke); free(ufl); free(vfl);
 }
 
 /* wall-normal coordinates of the profile slots (statistics.c:319-329) */
@@ -454,6 +485,7 @@ int main(int argc, char **argv) {
             if (!(ss % (50 * ut))) checkpoint(h, &d, TM + (ss / ut), d0.force);
             vel_print(h, &d, TM + (ss / ut));
             if (spectra_out) spectra_print(h, &d, TM + (ss / ut));
+            if (budget_out) budget_print(h, &d, TM + (ss / ut));
             statistics(h, &d, &p, TM + (ss / ut));
         }
         /* run up to the next output step (or the end) in one batch */

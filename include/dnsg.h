@@ -271,6 +271,22 @@ int dnsg_velocity_snapshot(dnsg_t *h, int grid, double *vel, double *den);
  * with N0, N1 the grid's streamwise and spanwise extents, nz = zdc or zdf. */
 int dnsg_spectra(dnsg_t *h, int grid, double *ex, double *ey);
 
+/* TKE-budget/computations (budget2D.c:13-1736, uv_fluc.c:14-46; the reference's
+ * offline budget tool, run here on the device from the current state): the
+ * per-snapshot turbulence statistics and kinetic-energy budget profiles, NZ values
+ * each (the profile layout of dnsg_statistics):
+ *   turb [19][NZ]  UAVG VAVG WAVG UTIN VTIN WTIN uw DAVG DTIN SKEWu SKEWv SKEWw
+ *                  SKEWuw SKEWp KURTu KURTv KURTw KURTuw KURTp (turb-fld-2d.000.NNNN)
+ *   vort [3][NZ]   rms vorticity fluctuations wx wy wz       (vort-fluc-2d.000.NNNN)
+ *   ke   [30][NZ]  Pii Puu Pvv Pww Puw Tiir Tuur Tvvr Twwr Tuwr Tiipai Tuupai Tvvpai
+ *                  Twwpai Tuwpai Paiuu Paivv Paiww Paiuw Tiis Tuus Tvvs Twws Tuws
+ *                  Epii Epuu Epvv Epww Epuw KE               (KE-Budget-2d.000.NNNN)
+ * and, if ufl and vfl are both given, the lower band's u' and v' at k = 4, [Xf][Yf]
+ * (ufl4/vfl4.NNNN).  The viscosity comes from Rebs and ub as in the tool.  The
+ * layer sums are taken in another order than the tool's sequential loops.
+ * Needs ydv >= 4 and zdv >= 6. */
+int dnsg_budget2d(dnsg_t *h, double *turb, double *vort, double *ke, double *ufl, double *vfl);
+
 /* ---- serving queue: steady slip-length solves, continuously batched -----------
  * No counterpart in the reference, which runs one channel per MPI job.  It is
  * for parametric sweeps: many independent channels of uneven sizes, each driven

@@ -65,6 +65,8 @@ def oracle_lib():
                                          ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         lib.orc_uniform_field.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_double,
                                           ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+        D8 = ctypes.POINTER(ctypes.c_double)
+        lib.orc_budget2d.argtypes = [ctypes.POINTER(dnsg.Params)] + [D8] * 11
         lib.orc_prng_sym.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64]
         lib.orc_prng_sym.restype = ctypes.c_double
         lib.orc_prng_bits.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64]
@@ -182,3 +184,43 @@ def uniform_field(xdu, ydu, zdu, seed=77, u_max=0.05, noise=0.1):
     out = np.empty((2, xdu, ydu, zdu + 2, 19))
     oracle_lib().orc_uniform_field(xdu, ydu, zdu, seed, u_max, noise, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     return out
+
+
+def oracle_budget(params, snaps, planes=False):
+    """the oracle's TKE-budget profiles of one snapshot {grid: (vel, den)} (grids 0, 1, 2)"""
+    p = dnsg.Params.from_buffer_copy(params)
+    NZ = 2 * (p.zds + 1) + p.zdv - 4
+    turb, vort, ke = np.empty((19, NZ)), np.empty((3, NZ)), np.empty((30, NZ))
+    Xf, Yf = p.gratio * p.xdv, p.gratio * p.ydv
+    uv = (np.empty((Xf, Yf)), np.empty((Xf, Yf))) if planes else (None, None)
+    P = lambda a: None if a is None else np.ascontiguousarray(a).ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    arrs = [np.ascontiguousarray(snaps[g][c]) for g, c in ((0, 0), (0, 1), (1, 0), (1, 1), (2, 0), (2, 1))]
+    rc = oracle_lib().orc_budget2d(ctypes.byref(p), *[P(a) for a in arrs], P(turb), P(vort), P(ke), P(uv[0]), P(uv[1]))
+    assert rc == 0
+    return turb, vort, ke, (uv if planes else None)
+
+
+def budget_scales(kind, ref):
+    """the scale each budget profile is compared on: the row's own magnitude for
+    UAVG, DAVG and vorticity, the mean velocity or density for the other means and
+    the rms values; for terms that are sums of signed products
+    (they can cancel to far below their summands) the scale of what they are
+    made of -- uw the rms product, skewness and kurtosis 1 (dimensionless), every
+    kinetic-energy budget term the largest term of the budget.  The density
+    skewness and kurtosis are moments normalised by the rms density fluctuation;
+    in a state whose density fluctuations are at rounding level (DTIN below 1e-12
+    of the mean) they are ratios of rounding noise and are not compared (scale inf)."""
+    mag = np.max(np.abs(ref), axis=1)
+    if kind == "turb":
+        s = mag.copy()
+        s[[1, 2]] = mag[0]                          # VAVG, WAVG against UAVG
+        s[3:6] = np.maximum(mag[3:6], mag[0])       # rms velocities against UAVG
+        s[8] = max(mag[8], mag[7])                  # rms density against DAVG
+        s[6] = np.max(ref[3]) * np.max(ref[5])      # uw against UTIN * WTIN
+        s[9:] = np.maximum(mag[9:], 1.0)            # skewness, kurtosis
+        if np.max(ref[8]) < 1e-12 * np.max(ref[7]):
+            s[[13, 18]] = np.inf                    # SKEWp, KURTp of rounding noise
+        return s
+    if kind == "ke":
+        return np.full(ref.shape[0], np.max(mag))
+    return mag

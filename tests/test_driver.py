@@ -237,6 +237,33 @@ def test_driver_writes_spectra(tmp_path, oracle_built):
                 assert np.max(np.abs(ey - ry[c])) <= 1e-12 * max(np.max(np.abs(ry)), 1e-300), (tag, name)
 
 
+@pytest.mark.gpu
+def test_driver_writes_budget(tmp_path, oracle_built):
+    """budget=1: the TKE-budget tool's files of every snapshot, against the oracle's
+    budget pass over the velocity/density files written beside them"""
+    kw = dict(MICRO, **SEEDED, force_mode=1, g_const="2e-5", ITER=3)
+    odir, gdir, _ = _run_pair(tmp_path, kw, "p")
+    r = subprocess.run([DRIVER, "-p"] + _args(kw) + ["results_dat=0", "budget=1"], cwd=gdir, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    post = gdir / "post"
+    post.mkdir()
+    subprocess.run([ORC_CLI, "budget", str(post)] + _args(kw) + ["TSTR=0", "TEND=3"], check=True)
+    from helpers import budget_scales, dnsg
+    d_nz = dnsg.dims_of(dnsg.make_params(**MICRO)).NZ
+    names = sorted(os.listdir(post))
+    assert len(names) == 3 * 4 + 2
+    for name in names:
+        a = np.fromfile(gdir / name)
+        b = np.fromfile(post / name)
+        assert a.shape == b.shape, name
+        kind = "plane" if name.startswith(("ufl", "vfl")) else {"tu": "turb", "vo": "vort", "KE": "ke"}[name[:2]]
+        shape = (1, -1) if kind == "plane" else (-1, d_nz)
+        a, b = a.reshape(shape), b.reshape(shape)
+        scale = budget_scales(kind, b) + 1e-300
+        assert np.all(np.max(np.abs(a - b), axis=1) <= 1e-10 * scale), name
+
+
 def test_driver_rejects_unknown_key(tmp_path):
     if not os.path.exists(DRIVER):
         pytest.skip("driver not built")
