@@ -283,7 +283,7 @@ def main():
         flux = {"glups": round(r["glups"], 4), "ms_per_step": round(r["ms_per_step"], 4),
                 "kernels_ms": {a: round(b, 4) for a, b in r["kernels_ms"].items()}}
         print(f"bench: fp64 flux {r['glups']:.3f} GLUPS", file=sys.stderr, flush=True)
-    probe = {k: round(mod.hbm_probe(4 << 30, i, 5), 1) for i, k in enumerate(("copy", "read", "write"))}
+    probe = {k: round(mod.hbm_probe(4 << 30, i, 5), 1) for i, k in enumerate(("copy", "read", "write", "lattice"))}
     print(f"bench: HBM probe {probe} GB/s", file=sys.stderr, flush=True)
     main_t = res["fp64"] if "fp64" in res else res[tiers[0]]
     main_prec = "fp64" if "fp64" in res else tiers[0]

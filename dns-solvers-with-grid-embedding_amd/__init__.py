@@ -399,7 +399,8 @@ def solve_bytes(params):
 
 
 def hbm_probe(nbytes=2 << 30, kind=0, reps=5):
-    """achievable HBM bandwidth (GB/s): kind 0 copy, 1 read, 2 write"""
+    """achievable HBM bandwidth (GB/s): kind 0 copy, 1 read, 2 write, 3 the
+    collide-stream's data movement without its arithmetic"""
     g = ctypes.c_double()
     rc = load().dnsg_hbm_probe(nbytes, kind, reps, ctypes.byref(g))
     if rc:

@@ -1,10 +1,13 @@
 // k_probe.hip -- HBM calibration kernels: a plain copy, a read-only sum and a
-// write-only fill over a buffer of known size.  They give the achievable
+// write-only fill over a buffer of known size, and the collide-stream's data
+// movement without its arithmetic (19 row blocks of a lattice row read, each
+// written into the row of its (y, z) neighbour).  They give the achievable
 // bandwidth the collide-stream kernels are compared with, and the reference
 // point that scales the FETCH_SIZE / WRITE_SIZE counters of rocprofv3.
 #include <string.h>
 
 #include "internal.hpp"
+#include "lattice.hpp"
 
 namespace dnsg {
 
@@ -30,12 +33,38 @@ __global__ __launch_bounds__(256) void k_probe_fill(double2 *__restrict__ dst, s
     if (i < n) dst[i] = make_double2(v, v);
 }
 
+// the lattice pattern: rows of nx = 1024 nodes in the row-interleaved layout, two
+// nodes per thread (256 apart), direction q stored into row (y + e_y, z + e_z)
+// with non-temporal stores (the best of the plain forms of this pattern measured)
+constexpr int PS_NX = 1024, PS_NY = 512;
+__global__ __launch_bounds__(256) void k_probe_lattice(const double *__restrict__ a, double *__restrict__ b, int nz) {
+    constexpr int per = 512, chunks = PS_NX / per;
+    const int row = blockIdx.x / chunks, x0 = (blockIdx.x % chunks) * per + threadIdx.x;
+    if (row >= PS_NY * nz) return;
+    const int y = row % PS_NY, z = row / PS_NY;
+    double v[2][Q];
+    unroll<0, Q>([&](auto qq) {
+        constexpr int q = decltype(qq)::value;
+        v[0][q] = a[lat_at(q, row, x0, PS_NX)];
+        v[1][q] = a[lat_at(q, row, x0 + 256, PS_NX)];
+    });
+    unroll<0, Q>([&](auto qq) {
+        constexpr int q = decltype(qq)::value;
+        const int yy = (y + EY[q] + PS_NY) % PS_NY, zz = (z + EZ[q] + nz) % nz;
+        const size_t drow = (size_t)zz * PS_NY + yy;
+        __builtin_nontemporal_store(v[0][q], b + lat_at(q, drow, x0, PS_NX));
+        __builtin_nontemporal_store(v[1][q], b + lat_at(q, drow, x0 + 256, PS_NX));
+    });
+}
+
 }  // namespace dnsg
 
 using namespace dnsg;
 
 extern "C" int dnsg_hbm_probe(size_t bytes, int kind, int reps, double *gbs) {
-    if (bytes < (1 << 20) || reps < 1 || !gbs || kind < 0 || kind > 2) return DNSG_ERR_ARG;
+    if (bytes < (1 << 20) || reps < 1 || !gbs || kind < 0 || kind > 3) return DNSG_ERR_ARG;
+    const int lnz = (int)(bytes / ((size_t)Q * PS_NX * PS_NY * 8));  // layers of the lattice pattern
+    if (kind == 3 && lnz < 3) return DNSG_ERR_ARG;
     const size_t n = bytes / sizeof(double2);
     double2 *a = nullptr, *b = nullptr;
     double *o = nullptr;
@@ -55,6 +84,9 @@ extern "C" int dnsg_hbm_probe(size_t bytes, int kind, int reps, double *gbs) {
             (void)hipEventRecord(e0, s);
             if (kind == 0) hipLaunchKernelGGL(k_probe_copy, grd, blk, 0, s, a, b, n);
             else if (kind == 1) hipLaunchKernelGGL(k_probe_read, grd, blk, 0, s, a, o, n);
+            else if (kind == 3)
+                hipLaunchKernelGGL(k_probe_lattice, dim3((unsigned)(PS_NY * lnz * (PS_NX / 512))), blk, 0, s,
+                                   (const double *)a, (double *)b, lnz);
             else hipLaunchKernelGGL(k_probe_fill, grd, blk, 0, s, b, n, 3.0);
             (void)hipEventRecord(e1, s);
             if (hipEventSynchronize(e1) != hipSuccess) rc = DNSG_ERR_HIP;
@@ -62,7 +94,8 @@ extern "C" int dnsg_hbm_probe(size_t bytes, int kind, int reps, double *gbs) {
             (void)hipEventElapsedTime(&ms, e0, e1);
             if (r > 0 && ms < best) best = ms;  // the first launch warms up
         }
-        const double moved = (kind == 0 ? 2.0 : 1.0) * (double)(n * sizeof(double2));
+        const double moved = kind == 3 ? 2.0 * Q * PS_NX * PS_NY * (double)lnz * 8
+                                       : (kind == 0 ? 2.0 : 1.0) * (double)(n * sizeof(double2));
         *gbs = moved / (best * 1e-3) / 1e9;
     }
     if (e0) (void)hipEventDestroy(e0);

@@ -215,7 +215,10 @@ int dnsg_kernel_times(dnsg_t *h, int n, double *ms);
 
 /* Instrumentation (no counterpart in the reference): achievable HBM bandwidth of
  * a plain kernel over a buffer of `bytes` -- kind 0 copy (read + write),
- * 1 read-only, 2 write-only -- best of `reps` launches, in GB/s. */
+ * 1 read-only, 2 write-only, 3 the collide-stream's data movement without its
+ * arithmetic (FP64 lattice rows of 1024 x 512 x nz, 19 directions read and each
+ * written into its (y, z) neighbour row; needs bytes >= 3 such layers) -- best of
+ * `reps` launches, in GB/s of bytes read plus written. */
 int dnsg_hbm_probe(size_t bytes, int kind, int reps, double *gbs);
 
 /* ---- reference-granular entry points (one per fun_defs.h function) ---------- */
