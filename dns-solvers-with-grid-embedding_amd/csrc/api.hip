@@ -576,8 +576,10 @@ static int sync_fwall(dnsg_t *h) {
 
 }  // extern "C"
 
+// once allocated, fwall is refreshed by every call that sets the state
 int dnsg::ensure_fwall(dnsg_t *h) {
-    if (!h->fwall) HIPCHK(h, hipMalloc(&h->fwall, fltr_wall_doubles(h) * 8));
+    if (h->fwall) return DNSG_OK;
+    HIPCHK(h, hipMalloc(&h->fwall, fltr_wall_doubles(h) * 8));
     return sync_fwall(h);
 }
 
@@ -1011,33 +1013,26 @@ int dnsg_stats_reset(dnsg_t *h) {
 }
 
 int dnsg_velocity_snapshot(dnsg_t *h, int grid, double *vel, double *den) {
-    if (!h || grid < 0 || grid > 2) return DNSG_ERR_ARG;
+    if (!h || !h->stream || grid < 0 || grid > 2) return DNSG_ERR_ARG;
+    if (!h->have_state) {
+        h->err = "no state: call dnsg_init_state or dnsg_upload_state first";
+        return DNSG_ERR_STATE;
+    }
+    // vel_print.c:86-117 on the device, then one copy of 4 values per node
+    int rc = ensure_fwall(h);
+    if (rc) return rc;
     const int X = grid == 0 ? h->d.Xc : h->d.Xf, Y = grid == 0 ? h->d.Yc : h->d.Yf;
     const int Z = grid == 0 ? h->d.zdc : h->d.zdf;
     const size_t nn = (size_t)X * Y * Z;
-    std::vector<double> aos(nn * 19);
-    int rc;
-    if (grid == 0) rc = dnsg_download_state(h, aos.data(), nullptr, nullptr, nullptr, nullptr, nullptr);
-    else if (grid == 1) rc = dnsg_download_state(h, nullptr, aos.data(), nullptr, nullptr, nullptr, nullptr);
-    else rc = dnsg_download_state(h, nullptr, nullptr, aos.data(), nullptr, nullptr, nullptr);
-    if (rc) return rc;
-    static const double E[3][19] = {{0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, 1, -1, -1, 0, 0, 0, 0},
-                                    {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, 1, -1, -1},
-                                    {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1}};
-    for (size_t m = 0; m < nn; m++) {  // vel_print.c:86-117
-        const double *f = aos.data() + m * 19;
-        double ux = 0., uy = 0., uz = 0., dens = 0.;
-        for (int a = 0; a < 19; a++) {
-            ux += E[0][a] * f[a];
-            uy += E[1][a] * f[a];
-            uz += E[2][a] * f[a];
-            dens += f[a];
-        }
-        if (dens > 0.) { ux /= dens; uy /= dens; uz /= dens; }
-        else { ux = 0.; uy = 0.; uz = 0.; dens = 0.; }
-        if (vel) { vel[3 * m] = ux; vel[3 * m + 1] = uy; vel[3 * m + 2] = uz; }
-        if (den) den[m] = dens;
-    }
+    double *dv = nullptr;
+    HIPCHK(h, hipMalloc(&dv, nn * 4 * 8));
+    hipError_t e = launch_snapshot(h, grid, 0, dv, dv + 3 * nn);
+    if (e == hipSuccess && vel) e = hipMemcpyAsync(vel, dv, nn * 3 * 8, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && den) e = hipMemcpyAsync(den, dv + 3 * nn, nn * 8, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    else (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(dv);
+    HIPCHK(h, e);
     return DNSG_OK;
 }
 

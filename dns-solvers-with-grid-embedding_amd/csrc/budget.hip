@@ -63,17 +63,18 @@ __device__ __forceinline__ double vat(const BudGrid &G, int i, int j, int k, int
     return G.vel[bidx(G, i, j, k) * 3 + c];
 }
 
-// 1. velocity and density as vel_print.c:86-117 writes them; the band wall layers
-//    come from the wall-layer copies [X][Y][19] (full f)
+// 1. velocity and density as vel_print.c:86-117 writes them, [X+2 pad][Y][Z] with
+//    pad periodic ghost planes on each side in x (pad 0: the vel-* / den-* file
+//    layout); the band wall layers come from the wall-layer copies [X][Y][19]
 template <class S>
 __global__ __launch_bounds__(BT) void k_bud_fill(const S *__restrict__ lat, const double *__restrict__ wall, int X,
-                                                 int Y, int Z, int zlo, int kwall, double *__restrict__ vel,
+                                                 int Y, int Z, int zlo, int kwall, int pad, double *__restrict__ vel,
                                                  double *__restrict__ den) {
-    const long n = (long)(X + 4) * Y * Z;
+    const long n = (long)(X + 2 * pad) * Y * Z;
     const long id = (long)blockIdx.x * BT + threadIdx.x;
     if (id >= n) return;
     const int k = (int)(id % Z), j = (int)((id / Z) % Y), p = (int)(id / ((long)Y * Z));
-    const int x = (p - 2 + X) % X;
+    const int x = (p - pad + X) % X;
     double f[Q];
     if (k == kwall) {
         const double *w = wall + ((size_t)x * Y + j) * Q;
@@ -431,6 +432,28 @@ void bud_finish(const dnsg_params &p, const double *Um, const double *dUdz, std:
 
 }  // namespace
 
+// velocity and density of grid g (0 coarse, 1 lower, 2 upper) from the current
+// state into [X + 2 pad][Y][Z][3] and [X + 2 pad][Y][Z]; needs fwall
+hipError_t launch_snapshot(dnsg_ctx *h, int g, int pad, double *vel, double *den) {
+    const dnsg_dims &d = h->d;
+    const Grid *lat = g == 0 ? &h->coarse : (g == 1 ? &h->lower : &h->upper);
+    const int X = g == 0 ? d.Xc : d.Xf, Y = g == 0 ? d.Yc : d.Yf, Z = g == 0 ? d.zdc : d.zdf;
+    const int zlo = g == 1 ? 1 : 0, kwall = g == 0 ? -1 : (g == 1 ? 0 : d.zdf - 1);
+    const double *wall = h->fwall + (g == 2 ? (size_t)d.Xf * d.Yf * 19 : 0);
+    const long n = (long)(X + 2 * pad) * Y * Z;
+    const dim3 grd((unsigned)((n + BT - 1) / BT)), blk(BT);
+    if (h->p.precision == DNSG_PREC_FP64)
+        hipLaunchKernelGGL(k_bud_fill<double>, grd, blk, 0, h->stream, (const double *)lat->c(), wall, X, Y, Z, zlo,
+                           kwall, pad, vel, den);
+    else if (h->p.precision == DNSG_PREC_FP32)
+        hipLaunchKernelGGL(k_bud_fill<float>, grd, blk, 0, h->stream, (const float *)lat->c(), wall, X, Y, Z, zlo,
+                           kwall, pad, vel, den);
+    else
+        hipLaunchKernelGGL(k_bud_fill<__half>, grd, blk, 0, h->stream, (const __half *)lat->c(), wall, X, Y, Z, zlo,
+                           kwall, pad, vel, den);
+    return hipGetLastError();
+}
+
 }  // namespace dnsg
 
 using namespace dnsg;
@@ -463,13 +486,10 @@ extern "C" int dnsg_budget2d(dnsg_t *h, double *turb, double *vort, double *ke, 
     A.GR = GR;
     A.coef = (1. / ((double)GR));
     A.NZ = NZ;
-    const Grid *lat[3] = {&h->coarse, &h->lower, &h->upper};
     const int X[3] = {d.Xc, d.Xf, d.Xf}, Y[3] = {d.Yc, d.Yf, d.Yf}, Z[3] = {zdc, zdf, zdf};
-    const int zlo[3] = {0, 1, 0}, kwall[3] = {-1, 0, zdf - 1};
     const int k0[3] = {2, 1, 0}, k1[3] = {zdc - 2, zdf, zdf - 1};    // node passes
     const int m0[3] = {2, 0, 0}, m1[3] = {zdc - 2, zdf, zdf};        // means and subtraction
     const int base[3] = {zdf - 2, 0, zdf + zdc - 4};
-    const size_t nw = (size_t)d.Xf * d.Yf * 19;
     size_t npart = 0;
     std::vector<double> mean(4 * (size_t)NZ, 0.), acc((size_t)NACC * NZ, 0.), dUdz(NZ, 0.), lay;
     double *dmean = nullptr, *dpart = nullptr, *dsum = nullptr, *dpl = nullptr;
@@ -487,19 +507,7 @@ extern "C" int dnsg_budget2d(dnsg_t *h, double *turb, double *vort, double *ke, 
     BCHK(h, hipMalloc(&dsum, (size_t)NACC * NZ * 8));
     // 1. snapshot arrays
     for (int g = 0; g < 3; g++) {
-        const long n = (long)(X[g] + 4) * Y[g] * Z[g];
-        const dim3 grd((unsigned)((n + BT - 1) / BT)), blk(BT);
-        const double *wall = g == 1 ? h->fwall : h->fwall + nw;
-        if (h->p.precision == DNSG_PREC_FP64)
-            hipLaunchKernelGGL(k_bud_fill<double>, grd, blk, 0, h->stream, (const double *)lat[g]->c(), wall, X[g],
-                               Y[g], Z[g], zlo[g], kwall[g], A.g[g].vel, A.g[g].den);
-        else if (h->p.precision == DNSG_PREC_FP32)
-            hipLaunchKernelGGL(k_bud_fill<float>, grd, blk, 0, h->stream, (const float *)lat[g]->c(), wall, X[g],
-                               Y[g], Z[g], zlo[g], kwall[g], A.g[g].vel, A.g[g].den);
-        else
-            hipLaunchKernelGGL(k_bud_fill<__half>, grd, blk, 0, h->stream, (const __half *)lat[g]->c(), wall, X[g],
-                               Y[g], Z[g], zlo[g], kwall[g], A.g[g].vel, A.g[g].den);
-        BCHK(h, hipGetLastError());
+        BCHK(h, launch_snapshot(h, g, 2, A.g[g].vel, A.g[g].den));
     }
     // 2. layer means
     for (int g = 0; g < 3; g++) {

@@ -191,8 +191,12 @@ hipError_t launch_ftcd_many(hipStream_t s, dnsg_ctx *const *hs, int n, double *p
 // memory) onto the three state lattices, and the check that its stencils stay inside
 int uniform_check(const dnsg_ctx *h, int xdu, int ydu, int zdu, std::string *err);
 hipError_t launch_uniform(hipStream_t s, dnsg_ctx *h, const double *u, int xdu, int ydu, int zdu);
-// device copies of the band wall layers (fwall), allocated on first use and
-// refreshed from the host copies
+// velocity and density of grid g (0 coarse, 1 lower band, 2 upper band) as
+// vel_print forms them, into [X + 2 pad][Y][Z][3] and [X + 2 pad][Y][Z] with pad
+// periodic ghost planes in x (0: the vel-* / den-* file layout); needs fwall
+hipError_t launch_snapshot(dnsg_ctx *h, int g, int pad, double *vel, double *den);
+// device copies of the band wall layers (fwall), allocated and filled from the
+// host copies on first use; every call that sets the state refreshes them
 int ensure_fwall(dnsg_t *h);
 // doubles of the filtered injection's f_neq scratch and of the wall-layer copies
 size_t fltr_fneq_doubles(const dnsg_ctx *h);

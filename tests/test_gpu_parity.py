@@ -237,6 +237,33 @@ def test_flux_controller_from_writes(name):
         assert relmax(ga[k], gb[k]) <= 1e-12, k
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_velocity_snapshot_is_vel_print(prec):
+    """dnsg_velocity_snapshot (formed on the device) against vel_print.c:86-117
+    evaluated in its own operation order on the downloaded state, wall layers
+    included: bit for bit"""
+    kw = dict(CASES["rslip"], force_mode="const", g_const=2e-5, precision=prec)
+    _, s = pair(kw)
+    s.step(4)
+    st = s.download()
+    E = np.array([[0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, 1, -1, -1, 0, 0, 0, 0],
+                  [0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, 1, -1, -1],
+                  [0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1]], dtype=float)
+    for g, key in enumerate(GRIDS):
+        f = st[key]
+        u = [np.zeros(f.shape[:3]) for _ in range(3)]
+        dens = np.zeros(f.shape[:3])
+        for a in range(19):
+            for c in range(3):
+                u[c] = u[c] + E[c, a] * f[..., a]
+            dens = dens + f[..., a]
+        ok = dens > 0.0
+        vel = np.stack([np.where(ok, uc / np.where(ok, dens, 1.0), 0.0) for uc in u], axis=-1)
+        vg, dg = s.velocity(g)
+        assert np.array_equal(vg, vel), key
+        assert np.array_equal(dg, np.where(ok, dens, 0.0)), key
+
+
 @pytest.mark.parametrize("name", ["pslip", "gr3"])
 def test_mass_and_statistics(name):
     o, s = pair(CASES[name])
