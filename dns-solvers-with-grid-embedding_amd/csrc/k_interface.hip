@@ -166,12 +166,30 @@ __device__ __forceinline__ void bicubic_coef(const P *xv, P *c) {
     });
 }
 
-// Horner form of ctfdtransfer_opt.c:59-62
-template <class P>
-__device__ __forceinline__ P bicubic_eval(const P *c, P tt, P uu) {
-    P r = P(0);
-    for (int nn = 3; nn > -1; nn--) r = tt * r + ((c[nn * 4 + 3] * uu + c[nn * 4 + 2]) * uu + c[nn * 4 + 1]) * uu + c[nn * 4 + 0];
-    return r;
+// Horner form of ctfdtransfer_opt.c:59-62 at the fine point (t, u) = (ii, jj)/GR
+// of a cell.  The reference's line groups as r = (t r + X u) + c0 with
+// X = (c3 u + c2) u + c1; that grouping is kept.  Where t or u is 0 the products
+// with it are dropped: for finite coefficients they only add a signed zero, which
+// leaves every nonzero result bit-identical (an exactly zero f_neq or velocity
+// gives the same stored f, f = feq + rescale f_neq with feq > 0).
+template <int GR, int ii, int jj, class P>
+__device__ __forceinline__ P bicubic_eval(const P *c) {
+    const P tt = (P)(((double)ii) / ((double)GR)), uu = (P)(((double)jj) / ((double)GR));
+    auto xu = [&](int nn) { return ((c[nn * 4 + 3] * uu + c[nn * 4 + 2]) * uu + c[nn * 4 + 1]) * uu; };
+    if constexpr (ii == 0) {
+        if constexpr (jj == 0) return c[0];
+        else return xu(0) + c[0];
+    } else {
+        P r;
+        if constexpr (jj == 0) {
+            r = c[12];
+            for (int nn = 2; nn > -1; nn--) r = tt * r + c[nn * 4];
+        } else {
+            r = xu(3) + c[12];
+            for (int nn = 2; nn > -1; nn--) r = (tt * r + xu(nn)) + c[nn * 4];
+        }
+        return r;
+    }
 }
 
 __constant__ int c_EX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, 1, -1, -1, 0, 0, 0, 0};
@@ -322,11 +340,10 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
             xv[2] = u0_[cy + 3][cx + 3]; xv[6] = u1_[cy + 1][cx + 1]; xv[10] = u2_[cy + 1][cx + 3]; xv[14] = u3_[cy + 1][cx + 1];
             xv[3] = u0_[cy + 3][cx + 2]; xv[7] = u1_[cy + 1][cx];     xv[11] = u2_[cy + 1][cx + 2]; xv[15] = u3_[cy + 1][cx];
             bicubic_coef(xv, c);
-#pragma unroll
-            for (int pp = 0; pp < NP; pp++) {
-                const int ii = pp / GR, jj = pp % GR;
-                r[pp] = bicubic_eval(c, (P)(((double)ii) / ((double)GR)), (P)(((double)jj) / ((double)GR)));
-            }
+            unroll<0, NP>([&](auto pc) {
+                constexpr int pp = decltype(pc)::value;
+                r[pp] = bicubic_eval<GR, pp / GR, pp % GR>(c);
+            });
         }
         if (v < 4) {
 #pragma unroll
