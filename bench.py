@@ -291,7 +291,7 @@ def main():
     traffic = traffic_from_profiles()
     roof = dict(bound="hbm", achieved=bl["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=(bl["gbs"] / HBM_PEAK_GBS) if bl["gbs"] else None, traffic=traffic,
-                kernel="k_collide_stream<double> (both wall bands, one launch per substep)",
+                kernel="k_collide_stream<double, false> (both wall bands, one launch per substep)",
                 algorithmic_bytes_per_launch=bl["bytes"], launch_ms=bl["ms"])
     prec_report = None
     if not args.no_precision_report:
