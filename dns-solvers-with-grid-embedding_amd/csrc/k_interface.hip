@@ -152,68 +152,67 @@ constexpr int WT[16][16] = {
     {-6, 6, -6, 6, -3, -3, 3, 3, -4, 4, 2, -2, -2, -2, -1, -1},
     {4, -4, 4, -4, 2, 2, -2, -2, 2, -2, -2, 2, 1, 1, 1, 1}};
 
-// c = WT x, summed in k order from 0 with the zero weights dropped
-template <class P>
-__device__ __forceinline__ void bicubic_coef(const P *xv, P *c) {
-    unroll<0, 16>([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        P xx = P(0);
-        unroll<0, 16>([&](auto kc) {
-            constexpr int k = decltype(kc)::value;
-            if constexpr (WT[i][k] != 0) xx = xx + (P)WT[i][k] * xv[k];
-        });
-        c[i] = xx;
-    });
+constexpr int wt_first(int i) {
+    for (int k = 0; k < 16; k++)
+        if (WT[i][k] != 0) return k;
+    return 16;
 }
 
-// Horner form of ctfdtransfer_opt.c:59-62 at the fine point (t, u) = (ii, jj)/GR
-// of a cell.  The reference's line groups as r = (t r + X u) + c0 with
-// X = (c3 u + c2) u + c1; that grouping is kept.  Where t or u is 0 the products
-// with it are dropped: for finite coefficients they only add a signed zero, which
-// leaves every nonzero result bit-identical (an exactly zero f_neq or velocity
-// gives the same stored f, f = feq + rescale f_neq with feq > 0).
-template <int GR, int ii, int jj, class P>
-__device__ __forceinline__ P bicubic_eval(const P *c) {
-    const P tt = (P)(((double)ii) / ((double)GR)), uu = (P)(((double)jj) / ((double)GR));
-    auto xu = [&](int nn) { return ((c[nn * 4 + 3] * uu + c[nn * 4 + 2]) * uu + c[nn * 4 + 1]) * uu; };
-    if constexpr (ii == 0) {
-        if constexpr (jj == 0) return c[0];
-        else return xu(0) + c[0];
-    } else {
-        P r;
-        if constexpr (jj == 0) {
-            r = c[12];
-            for (int nn = 2; nn > -1; nn--) r = tt * r + c[nn * 4];
-        } else {
-            r = xu(3) + c[12];
-            for (int nn = 2; nn > -1; nn--) r = (tt * r + xu(nn)) + c[nn * 4];
+// row i of c = WT x, summed in k order with the zero weights dropped.  The
+// reference's sum starts from 0 (bcucof_opt.c:52-55); starting from the first
+// term instead can change only the sign of an exactly zero partial sum, which no
+// nonzero result sees.
+template <int i, class P>
+__device__ __forceinline__ P wt_row(const P *xv) {
+    P xx;
+    unroll<0, 16>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if constexpr (WT[i][k] != 0) {
+            const P t = (P)WT[i][k] * xv[k];
+            if constexpr (k == wt_first(i)) xx = t;
+            else xx = xx + t;
         }
-        return r;
-    }
+    });
+    return xx;
 }
 
-__constant__ int c_EX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, 1, -1, -1, 0, 0, 0, 0};
-__constant__ int c_EY[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, 1, -1, -1};
-__constant__ int c_EZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
-__constant__ double c_WQ[19] = {1. / 3., 1. / 18., 1. / 18., 1. / 18., 1. / 18., 1. / 18., 1. / 18.,
-                                1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36.,
-                                1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36.};
-
-// e_q . u for a runtime direction, with the same operation order as edot<q>
-template <class P>
-__device__ __forceinline__ P edot_rt(int q, P u0, P u1, P u2) {
-    P s = P(0);
-    bool have = false;
-    const int e[3] = {c_EX[q], c_EY[q], c_EZ[q]};
-    const P u[3] = {u0, u1, u2};
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        if (e[c] == 0) continue;
-        const P t = e[c] > 0 ? u[c] : -u[c];
-        s = have ? s + t : t;
-        have = true;
-    }
-    return s;
+// The bicubic patch of one cell evaluated at its GR x GR fine points (t, u) =
+// (ii, jj)/GR: coefficients c = WT x (bcucof_opt.c:47-61) and the Horner form of
+// ctfdtransfer_opt.c:59-62,
+//   r = c12 + X3 u, then r = (t r + X_nn u) + c_4nn for nn = 2, 1, 0,
+//   X_nn = (c_4nn+3 u + c_4nn+2) u + c_4nn+1,
+// with that grouping kept.  Each block of four coefficients is formed just before
+// the Horner step that takes it, so only one block is live at a time.  Where t or
+// u is 0 the products with it are dropped: for finite coefficients they only add a
+// signed zero, which leaves every nonzero result bit-identical (an exactly zero
+// f_neq or velocity gives the same stored f, f = feq + rescale f_neq with feq > 0).
+template <int GR, class P>
+__device__ __forceinline__ void bicubic_patch(const P *xv, P *r) {
+    unroll<0, 4>([&](auto bc) {
+        constexpr int nn = 3 - decltype(bc)::value;
+        P cb[4];
+        unroll<0, 4>([&](auto ic) {
+            constexpr int i = 4 * nn + decltype(ic)::value;
+            cb[decltype(ic)::value] = wt_row<i>(xv);
+        });
+        unroll<0, GR * GR>([&](auto pc) {
+            constexpr int pp = decltype(pc)::value, ii = pp / GR, jj = pp % GR;
+            const P tt = (P)(((double)ii) / ((double)GR)), uu = (P)(((double)jj) / ((double)GR));
+            auto xu = [&]() { return ((cb[3] * uu + cb[2]) * uu + cb[1]) * uu; };
+            if constexpr (ii == 0) {
+                if constexpr (nn == 0) {
+                    if constexpr (jj == 0) r[pp] = cb[0];
+                    else r[pp] = xu() + cb[0];
+                }
+            } else if constexpr (nn == 3) {
+                if constexpr (jj == 0) r[pp] = cb[0];
+                else r[pp] = xu() + cb[0];
+            } else {
+                if constexpr (jj == 0) r[pp] = tt * r[pp] + cb[0];
+                else r[pp] = (tt * r[pp] + xu()) + cb[0];
+            }
+        });
+    });
 }
 
 // two adjacent values of one row, one vector store (the first at an even index)
@@ -233,12 +232,13 @@ __device__ __forceinline__ void store_pair(__half *p, float a, float b) {
 //   1. forms the Hermite-in-time value on the tile plus a 2-cell halo, in LDS
 //      (the loads of the next value are issued before this value is processed);
 //   2. takes d/dx, d/dy at the tile's (TX+1) x (TY+1) corners (d/dy also on the
-//      halo columns) and d2/dxdy = d/dx of d/dy, in LDS;
-//   3. builds each cell's 16 bicubic coefficients from its 4 corners and
-//      evaluates them at the cell's GR x GR fine columns;
-//   4. keeps rho and u in registers; for a distribution value writes
-//      f = feq + tau_f/(GR tau_c) f_neq through an LDS tile so that the
-//      interface-layer rows are stored contiguously.
+//      halo columns), in LDS;
+//   3. forms d2/dxdy = d/dx of d/dy at its cell's 4 corners, builds the cell's 16
+//      bicubic coefficients and evaluates them at its GR x GR fine columns;
+//   4. keeps rho and u (the first four values, each a phase of its own with a
+//      fixed destination register set) in registers; for a distribution value
+//      writes f = feq + tau_f/(GR tau_c) f_neq straight from registers, GR
+//      adjacent fine values per store.
 struct IfaceItem {
     const void *ln, *lp, *lpp;  // newest, previous, oldest plane levels [23][Yc][Xc]
     void *band;                 // band post-stream lattice
@@ -252,12 +252,13 @@ template <class S, class P, int GR, int TX, int TY>
 __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<P> H) {
     constexpr int NT = TX * TY, HX = TX + 5, HY = TY + 5, NP = GR * GR;
     constexpr int NH = (HX * HY + NT - 1) / NT;  // halo points per thread
-    // two sets, used by alternate values: the next value's tiles are written while
-    // slower threads still read this value's
-    __shared__ P s0[2][HY][HX];          // value, halo 2 on each side (+1 corner)
-    __shared__ P s1[2][TY + 1][TX + 1];  // d/dx at the corners
-    __shared__ P s2[2][TY + 1][HX];      // d/dy at the corners and the halo columns
-    __shared__ P s3[2][TY + 1][TX + 1];  // d2/dxdy at the corners
+    // The value tile has two sets, used by alternate values: the next value is
+    // written there before the first wait, while slower threads still read this
+    // one.  The derivative tiles are written only after that wait, when every
+    // thread has finished reading them, so one set serves.
+    __shared__ P s0[2][HY][HX];       // value, halo 2 on each side (+1 corner)
+    __shared__ P s1[TY + 1][TX + 1];  // d/dx at the corners
+    __shared__ P s2[TY + 1][HX];      // d/dy at the corners and the halo columns
     const int itm = batch_find(BT.first, BT.n, blockIdx.x);
     const IfaceItem &B = BT.it[itm];
     const int lbg = blockIdx.x - BT.first[itm];
@@ -288,9 +289,9 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
 #pragma unroll
     for (int k = 0; k < NH; k++)
         if (hok[k]) { a[k][0] = Ln[hoff[k]]; a[k][1] = Lp[hoff[k]]; a[k][2] = Lpp[hoff[k]]; }
-    P dens[NP], u0[NP], u1[NP], u2[NP], vv[NP];
-    for (int iv = 0; iv < nv; iv++) {
-        const int v = iv < 4 ? iv : 4 + q0 + (iv - 4);
+    // value phase iv (value 0..3: rho, u; then directions q0..q1-1): steps 1-3 for
+    // that value, its bicubic result at the cell's fine columns into r
+    auto phase = [&](int iv, P *r) __attribute__((always_inline)) {
         const int sb = iv & 1;
         // 1. Hermite value on the halo tile (tinterpolate_hrmt.c:28-34)
 #pragma unroll
@@ -314,80 +315,82 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
         // 2. d/dy on corner rows for every halo column; d/dx at the corners (derivative.c:118-119)
         P(*t0)[HX] = s0[sb];
         for (int p = tid; p < (TY + 1) * HX; p += NT) {
-            const int r = p / HX, hx = p % HX, hy = r + 2;
-            s2[sb][r][hx] = d4(t0[hy - 2][hx], t0[hy - 1][hx], t0[hy + 1][hx], t0[hy + 2][hx]);
+            const int rr = p / HX, hx = p % HX, hy = rr + 2;
+            s2[rr][hx] = d4(t0[hy - 2][hx], t0[hy - 1][hx], t0[hy + 1][hx], t0[hy + 2][hx]);
             if (hx >= 2 && hx <= TX + 2)
-                s1[sb][r][hx - 2] = d4(t0[hy][hx - 2], t0[hy][hx - 1], t0[hy][hx + 1], t0[hy][hx + 2]);
-        }
-        // 3. d2/dxdy = d/dx of d/dy (derivative.c:139); the d/dy values are formed
-        //    again here, with the same expression, so no wait for the loop above
-        for (int p = tid; p < (TY + 1) * (TX + 1); p += NT) {
-            const int r = p / (TX + 1), c = p % (TX + 1), hx = c + 2, hy = r + 2;
-            auto dy = [&](int h) { return d4(t0[hy - 2][h], t0[hy - 1][h], t0[hy + 1][h], t0[hy + 2][h]); };
-            s3[sb][r][c] = d4(dy(hx - 2), dy(hx - 1), dy(hx + 1), dy(hx + 2));
+                s1[rr][hx - 2] = d4(t0[hy][hx - 2], t0[hy][hx - 1], t0[hy][hx + 1], t0[hy][hx + 2]);
         }
         __syncthreads();
-        // 4. bicubic patch of this cell (bcucof_opt.c:47-61) at its fine columns
+        // 3. bicubic patch of this cell (bcucof_opt.c:47-61) at its fine columns;
+        //    d2/dxdy = d/dx of d/dy (derivative.c:139) at the cell's corners, from
+        //    the d/dy of columns cx .. cx+5 on its two corner rows
+        P xv[16];
+        P dyr[2][6];
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int i = 0; i < 6; i++) dyr[j][i] = s2[cy + j][cx + i];
+        xv[0] = t0[cy + 2][cx + 2]; xv[4] = s1[cy][cx];         xv[8] = dyr[0][2];
+        xv[1] = t0[cy + 2][cx + 3]; xv[5] = s1[cy][cx + 1];     xv[9] = dyr[0][3];
+        xv[2] = t0[cy + 3][cx + 3]; xv[6] = s1[cy + 1][cx + 1]; xv[10] = dyr[1][3];
+        xv[3] = t0[cy + 3][cx + 2]; xv[7] = s1[cy + 1][cx];     xv[11] = dyr[1][2];
+        xv[12] = d4(dyr[0][0], dyr[0][1], dyr[0][3], dyr[0][4]);
+        xv[13] = d4(dyr[0][1], dyr[0][2], dyr[0][4], dyr[0][5]);
+        xv[14] = d4(dyr[1][1], dyr[1][2], dyr[1][4], dyr[1][5]);
+        xv[15] = d4(dyr[1][0], dyr[1][1], dyr[1][3], dyr[1][4]);
+        bicubic_patch<GR>(xv, r);
+    };
+    // rho and u stay in registers for every direction of this workgroup
+    P dens[NP], u0[NP], u1[NP], u2[NP], vv[NP];
+    phase(0, dens);
+    phase(1, u0);
+    phase(2, u1);
+    phase(3, u2);
+#pragma unroll
+    for (int pp = 0; pp < NP; pp++) vv[pp] = u0[pp] * u0[pp] + u1[pp] * u1[pp] + u2[pp] * u2[pp];
+    for (int iv = 4; iv < nv; iv++) {
         P r[NP];
-        {
-            P xv[16], c[16];
-            const P(*u0_)[HX] = s0[sb];
-            const P(*u1_)[TX + 1] = s1[sb];
-            const P(*u2_)[HX] = s2[sb];
-            const P(*u3_)[TX + 1] = s3[sb];
-            xv[0] = u0_[cy + 2][cx + 2]; xv[4] = u1_[cy][cx];         xv[8] = u2_[cy][cx + 2];      xv[12] = u3_[cy][cx];
-            xv[1] = u0_[cy + 2][cx + 3]; xv[5] = u1_[cy][cx + 1];     xv[9] = u2_[cy][cx + 3];      xv[13] = u3_[cy][cx + 1];
-            xv[2] = u0_[cy + 3][cx + 3]; xv[6] = u1_[cy + 1][cx + 1]; xv[10] = u2_[cy + 1][cx + 3]; xv[14] = u3_[cy + 1][cx + 1];
-            xv[3] = u0_[cy + 3][cx + 2]; xv[7] = u1_[cy + 1][cx];     xv[11] = u2_[cy + 1][cx + 2]; xv[15] = u3_[cy + 1][cx];
-            bicubic_coef(xv, c);
-            unroll<0, NP>([&](auto pc) {
-                constexpr int pp = decltype(pc)::value;
-                r[pp] = bicubic_eval<GR, pp / GR, pp % GR>(c);
-            });
-        }
-        if (v < 4) {
+        phase(iv, r);
+        // 4. f = feq + tau_f/(GR tau_c) f_neq for direction q (uniform over the
+        //    workgroup): one branch per direction, so e_q . u is edot<q> itself
+        const int q = q0 + iv - 4;
+        // feq depends on rho and u only: without this (empty) register constraint
+        // the compiler hoists it out of the loop for all 19 directions at once,
+        // which holds 19 NP values in registers and lowers occupancy
+#pragma unroll
+        for (int pp = 0; pp < NP; pp++) asm volatile("" : "+v"(dens[pp]), "+v"(u0[pp]), "+v"(u1[pp]), "+v"(u2[pp]), "+v"(vv[pp]));
+        unroll<0, Q>([&](auto qc) {
+            constexpr int QD = decltype(qc)::value;
+            if (q != QD) return;
+            const P wq = (P)WQ[QD];
+            P out[NP];
 #pragma unroll
             for (int pp = 0; pp < NP; pp++) {
-                if (v == 0) dens[pp] = r[pp];
-                else if (v == 1) u0[pp] = r[pp];
-                else if (v == 2) u1[pp] = r[pp];
-                else u2[pp] = r[pp];
-            }
-            if (v == 3) {
-#pragma unroll
-                for (int pp = 0; pp < NP; pp++) vv[pp] = u0[pp] * u0[pp] + u1[pp] * u1[pp] + u2[pp] * u2[pp];
-            }
-            continue;
-        }
-        const int q = v - 4;
-        const P wq = (P)c_WQ[q];
-        P out[NP];
-#pragma unroll
-        for (int pp = 0; pp < NP; pp++) {
-            const P eu = edot_rt(q, u0[pp], u1[pp], u2[pp]);
-            if constexpr (sizeof(S) == 8) {
-                const P feq = wq * dens[pp] * ((P)1. + (P)3. * eu + (P)4.5 * eu * eu - (P)1.5 * vv[pp]);
-                out[pp] = feq + rescale * r[pp];  // ctfdtransfer_opt.c:93
-            } else {  // dens holds rho - 1
-                const P rho = (P)1. + dens[pp];
-                const P geq = wq * (dens[pp] + rho * ((P)3. * eu + (P)4.5 * eu * eu - (P)1.5 * vv[pp]));
-                out[pp] = geq + rescale * r[pp];
-            }
-        }
-        // the fine columns of this cell, straight from registers: a half-wave's cells
-        // cover one contiguous run of each fine row
-        if (x0 + cx < Xc && y0 + cy < Yc) {
-#pragma unroll
-            for (int jj = 0; jj < GR; jj++) {
-                S *dst = band + lat_at(q, (size_t)B.layer * Yf + GR * (y0 + cy) + jj, GR * (x0 + cx), Xf);
-                if constexpr (GR == 2) {
-                    store_pair(dst, out[jj], out[GR + jj]);  // ii = 0, 1
-                } else {
-#pragma unroll
-                    for (int ii = 0; ii < GR; ii++) store_s(dst + ii, out[ii * GR + jj]);
+                const P eu = edot<QD>(u0[pp], u1[pp], u2[pp]);
+                if constexpr (sizeof(S) == 8) {
+                    const P feq = wq * dens[pp] * ((P)1. + (P)3. * eu + (P)4.5 * eu * eu - (P)1.5 * vv[pp]);
+                    out[pp] = feq + rescale * r[pp];  // ctfdtransfer_opt.c:93
+                } else {  // dens holds rho - 1
+                    const P rho = (P)1. + dens[pp];
+                    const P geq = wq * (dens[pp] + rho * ((P)3. * eu + (P)4.5 * eu * eu - (P)1.5 * vv[pp]));
+                    out[pp] = geq + rescale * r[pp];
                 }
             }
-        }
+            // the fine columns of this cell, straight from registers: a half-wave's
+            // cells cover one contiguous run of each fine row
+            if (x0 + cx < Xc && y0 + cy < Yc) {
+#pragma unroll
+                for (int jj = 0; jj < GR; jj++) {
+                    S *dst = band + lat_at(QD, (size_t)B.layer * Yf + GR * (y0 + cy) + jj, GR * (x0 + cx), Xf);
+                    if constexpr (GR == 2) {
+                        store_pair(dst, out[jj], out[GR + jj]);  // ii = 0, 1
+                    } else {
+#pragma unroll
+                        for (int ii = 0; ii < GR; ii++) store_s(dst + ii, out[ii * GR + jj]);
+                    }
+                }
+            }
+        });
     }
 }
 
