@@ -17,6 +17,7 @@
 // Shifted tiers (lattices hold g = f - w): FP32 planes and arithmetic, slot 0
 // holds rho - 1 instead of rho, and f_neq = g - (feq - w).
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "internal.hpp"
@@ -126,7 +127,18 @@ hipError_t launch_sample_many(hipStream_t s, dnsg_ctx *const *hs, int n) {
 template <class P>
 struct HermiteW {
     P h00, h10, h01, h11;
+    int exact;  // every weight is 0 or a power of two (gratio 1 and 2): products exact
 };
+
+// a b + c with one rounding, used only where the product a b is exact (a is 0 or a
+// power of two): then it equals round(round(a b) + c), the reference's unfused form,
+// bit for bit, in one instruction
+__device__ __forceinline__ double fma_exact(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_exact(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+constexpr bool pow2_int(int w) { return w > 0 && (w & (w - 1)) == 0; }
+// i / g is a power of two (i > 0)
+constexpr bool frac_pow2(int i, int g) { return i > 0 && g % i == 0 && pow2_int(g / i); }
 
 template <class P>
 __device__ __forceinline__ P d4(P fm2, P fm1, P fp1, P fp2) {
@@ -167,10 +179,11 @@ __device__ __forceinline__ P wt_row(const P *xv) {
     P xx;
     unroll<0, 16>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        if constexpr (WT[i][k] != 0) {
-            const P t = (P)WT[i][k] * xv[k];
-            if constexpr (k == wt_first(i)) xx = t;
-            else xx = xx + t;
+        constexpr int w = WT[i][k], aw = w < 0 ? -w : w;
+        if constexpr (w != 0) {
+            if constexpr (k == wt_first(i)) xx = (P)w * xv[k];
+            else if constexpr (aw != 1 && pow2_int(aw)) xx = fma_exact((P)w, xv[k], xx);
+            else xx = xx + (P)w * xv[k];
         }
     });
     return xx;
@@ -197,19 +210,32 @@ __device__ __forceinline__ void bicubic_patch(const P *xv, P *r) {
         });
         unroll<0, GR * GR>([&](auto pc) {
             constexpr int pp = decltype(pc)::value, ii = pp / GR, jj = pp % GR;
+            constexpr bool t2 = frac_pow2(ii, GR), u2 = frac_pow2(jj, GR);  // exact products
             const P tt = (P)(((double)ii) / ((double)GR)), uu = (P)(((double)jj) / ((double)GR));
-            auto xu = [&]() { return ((cb[3] * uu + cb[2]) * uu + cb[1]) * uu; };
+            auto madd_u = [&](P a, P c) {  // a u + c
+                if constexpr (u2) return fma_exact(a, uu, c);
+                else return a * uu + c;
+            };
+            // X = (cb3 u + cb2) u + cb1, then X u + cb0
+            auto xu_plus = [&]() { return madd_u(madd_u(madd_u(cb[3], cb[2]), cb[1]), cb[0]); };
             if constexpr (ii == 0) {
                 if constexpr (nn == 0) {
                     if constexpr (jj == 0) r[pp] = cb[0];
-                    else r[pp] = xu() + cb[0];
+                    else r[pp] = xu_plus();
                 }
             } else if constexpr (nn == 3) {
                 if constexpr (jj == 0) r[pp] = cb[0];
-                else r[pp] = xu() + cb[0];
-            } else {
-                if constexpr (jj == 0) r[pp] = tt * r[pp] + cb[0];
-                else r[pp] = (tt * r[pp] + xu()) + cb[0];
+                else r[pp] = xu_plus();
+            } else if constexpr (jj == 0) {  // t r + cb0
+                if constexpr (t2) r[pp] = fma_exact(tt, r[pp], cb[0]);
+                else r[pp] = tt * r[pp] + cb[0];
+            } else {  // (t r + X u) + cb0
+                const P X = madd_u(madd_u(cb[3], cb[2]), cb[1]);
+                P m;
+                if constexpr (u2) m = fma_exact(X, uu, tt * r[pp]);
+                else if constexpr (t2) m = fma_exact(tt, r[pp], X * uu);
+                else m = tt * r[pp] + X * uu;
+                r[pp] = m + cb[0];
             }
         });
     });
@@ -302,8 +328,11 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
             const P c1 = lp;
             const P c2 = (ln - lpp) * (P)0.5;
             const P c3 = ln;
-            const P c4 = ((P)3. * ln - (P)4. * lp + lpp) * (P)0.5;
-            s0[sb][p / HX][p % HX] = H.h00 * c1 + H.h10 * c2 + H.h01 * c3 + H.h11 * c4;
+            const P c4 = (fma_exact((P)-4., lp, (P)3. * ln) + lpp) * (P)0.5;  // 3 ln - 4 lp
+            P hv;
+            if (H.exact) hv = fma_exact(H.h11, c4, fma_exact(H.h01, c3, fma_exact(H.h10, c2, H.h00 * c1)));
+            else hv = H.h00 * c1 + H.h10 * c2 + H.h01 * c3 + H.h11 * c4;
+            s0[sb][p / HX][p % HX] = hv;
         }
         if (iv + 1 < nv) {  // prefetch the next value
             const size_t vo = (size_t)(iv + 1 < 4 ? iv + 1 : 4 + q0 + (iv + 1 - 4)) * n;
@@ -443,6 +472,11 @@ hipError_t launch_interface_many(hipStream_t s, dnsg_ctx *const *hs, int n, int 
     H.h10 = (tmp * tmp * tmp - 2. * tmp * tmp + tmp);
     H.h01 = (-2. * tmp * tmp * tmp + 3. * tmp * tmp);
     H.h11 = (tmp * tmp * tmp - tmp * tmp);
+    auto pow2_or_zero = [](double w) {
+        int e;
+        return w == 0. || std::fabs(std::frexp(w, &e)) == 0.5;
+    };
+    H.exact = pow2_or_zero(H.h00) && pow2_or_zero(H.h10) && pow2_or_zero(H.h01) && pow2_or_zero(H.h11);
     std::vector<IfaceItem> v(2 * (size_t)n);
     long tiles = 0;
     for (int i = 0; i < n; i++) {
@@ -452,7 +486,7 @@ hipError_t launch_interface_many(hipStream_t s, dnsg_ctx *const *hs, int n, int 
     const int G = iface_groups(tiles);
     for (auto &it : v) it.groups = G;
     if (prec == DNSG_PREC_FP64) return iface_items<double, double>(s, GR, v.data(), 2 * n, H);
-    HermiteW<float> Hf = {(float)H.h00, (float)H.h10, (float)H.h01, (float)H.h11};
+    HermiteW<float> Hf = {(float)H.h00, (float)H.h10, (float)H.h01, (float)H.h11, H.exact};
     if (prec == DNSG_PREC_FP32) return iface_items<float, float>(s, GR, v.data(), 2 * n, Hf);
     return iface_items<__half, float>(s, GR, v.data(), 2 * n, Hf);
 }

@@ -27,6 +27,7 @@ CASES = {
     "pslip": dict(MICRO, pattern="pslip", gxplussize=4, wxplussize=4, gyplussize=4, wyplussize=4),
     "sslip": dict(MICRO, pattern="sslip", gxplussize=6, wxplussize=2),
     "gr3": dict(MICRO, gratio=3, zds=5, pattern="rslip", gyplussize=6, wyplussize=6),
+    "gr1": dict(MICRO, gratio=1, pattern="rslip", gyplussize=4, wyplussize=4),
     "ragged": dict(xdv=13, ydv=7, zdv=9, gratio=2, zds=5, Rebs=20.0, ub=0.03, CFL=5.0,
                    pattern="pslip", gxplussize=2, wxplussize=24, gyplussize=5, wyplussize=2),
     # rows of 256 (coarse) and 512 (bands) nodes: the collide-stream's aligned-store form
