@@ -15,7 +15,9 @@
 // FP64 tier: FP64 planes, every expression in the reference's operation order,
 // so the results are bitwise those of the reference.
 // Shifted tiers (lattices hold g = f - w): FP32 planes and arithmetic, slot 0
-// holds rho - 1 instead of rho, and f_neq = g - (feq - w).
+// holds rho - 1 instead of rho, and f_neq = g - (feq - w).  Their bicubic patch is
+// one compile-time weight matrix applied with FMAs instead of the reference's
+// coefficient-then-Horner order.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -199,8 +201,44 @@ __device__ __forceinline__ P wt_row(const P *xv) {
 // u is 0 the products with it are dropped: for finite coefficients they only add a
 // signed zero, which leaves every nonzero result bit-identical (an exactly zero
 // f_neq or velocity gives the same stored f, f = feq + rescale f_neq with feq > 0).
+// weight of x_k in the patch value at fine point pp: sum over the coefficients
+// c_(4a+b) = (WT x)_(4a+b) of t^a u^b, formed in double at compile time
+constexpr double bicubic_w(int GR, int pp, int k) {
+    const double t = double(pp / GR) / GR, u = double(pp % GR) / GR;
+    double acc = 0., ta = 1.;
+    for (int a = 0; a < 4; a++) {
+        double ub = 1.;
+        for (int b = 0; b < 4; b++) {
+            acc += ta * ub * WT[4 * a + b][k];
+            ub *= u;
+        }
+        ta *= t;
+    }
+    return (acc < 1e-12 && acc > -1e-12) ? 0. : acc;
+}
+constexpr int bicubic_first(int GR, int pp) {
+    for (int k = 0; k < 16; k++)
+        if (bicubic_w(GR, pp, k) != 0.) return k;
+    return 16;
+}
+
 template <int GR, class P>
 __device__ __forceinline__ void bicubic_patch(const P *xv, P *r) {
+    if constexpr (sizeof(P) == 4) {
+        // shifted tiers (FP32 arithmetic, compared within a tolerance): the patch is
+        // linear in x, so each fine value is one FMA chain over the nonzero weights
+        unroll<0, GR * GR>([&](auto pc) {
+            constexpr int pp = decltype(pc)::value, k0 = bicubic_first(GR, pp);
+            P acc = (P)bicubic_w(GR, pp, k0) * xv[k0];
+            unroll<k0 + 1, 16>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                constexpr float w = (float)bicubic_w(GR, pp, k);
+                if constexpr (w != 0.f) acc = __builtin_fmaf(w, xv[k], acc);
+            });
+            r[pp] = acc;
+        });
+        return;
+    }
     unroll<0, 4>([&](auto bc) {
         constexpr int nn = 3 - decltype(bc)::value;
         P cb[4];
