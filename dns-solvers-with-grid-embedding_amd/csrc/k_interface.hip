@@ -464,30 +464,62 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
 // Coarse tile of one workgroup.  The 19 directions can be shared out over G
 // workgroups per tile, each forming rho and u itself: that multiplies the work by
 // (4 + 19/G) G / 23 but shortens each workgroup's chain of sequential value phases
-// to 4 + 19/G.  A launch with enough tiles to fill the GPU (the Re_tau=180 bands:
-// 1024 tiles) is fastest with G = 1; the serving queue's small channels leave most
-// CUs idle and finish sooner with G = 4.
+// to 4 + 19/G.  G fills the workgroup slots the GPU holds at once for this kernel
+// (CUs x resident workgroups per CU, from the occupancy query): a launch whose
+// tiles alone fill them (the Re_tau=180 bands: 1024 tiles) takes G = 1, a launch of
+// small channels (the serving queue) shortens its chains, up to G = 19.
 constexpr int IF_TX = 32, IF_TY = 8;
-static int iface_groups(long tiles) { return tiles >= 1024 ? 1 : (tiles >= 512 ? 2 : 4); }
 
-template <class S, class P>
-static hipError_t iface_items(hipStream_t s, int GR, const IfaceItem *items, int n, HermiteW<P> H) {
+template <class S, class P, int GR>
+static int iface_slots() {
+    static int slots = 0;  // per kernel; concurrent first calls compute the same value
+    if (slots == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_iface<S, P, GR, IF_TX, IF_TY>, IF_TX * IF_TY, 0) ==
+                hipSuccess &&
+            cus > 0 && per_cu > 0)
+            slots = cus * per_cu;
+        else
+            slots = 1024;
+    }
+    return slots;
+}
+
+static int iface_groups(long tiles, int slots) {
+    if (tiles >= slots) return 1;
+    return (int)std::min<long>(Q, slots / tiles);
+}
+
+template <class S, class P, int GR>
+static hipError_t iface_launch(hipStream_t s, IfaceItem *items, int n, HermiteW<P> H) {
     for (int i0 = 0; i0 < n; i0 += BATCH_MAX) {
+        const int m = std::min(BATCH_MAX, n - i0);
+        long tiles = 0;
+        for (int i = i0; i < i0 + m; i++)
+            tiles += (long)((items[i].Xc + IF_TX - 1) / IF_TX) * ((items[i].Yc + IF_TY - 1) / IF_TY);
+        const int G = iface_groups(tiles, iface_slots<S, P, GR>());
+        for (int i = i0; i < i0 + m; i++) items[i].groups = G;
         Batch<IfaceItem> B;
-        const int blocks = fill_batch(B, items + i0, std::min(BATCH_MAX, n - i0), [](const IfaceItem &a) {
+        const int blocks = fill_batch(B, items + i0, m, [](const IfaceItem &a) {
             return ((a.Xc + IF_TX - 1) / IF_TX) * ((a.Yc + IF_TY - 1) / IF_TY) * a.groups;
         });
-        const dim3 grd(blocks), blk(IF_TX * IF_TY);
-        switch (GR) {
-        case 1: hipLaunchKernelGGL((k_iface<S, P, 1, IF_TX, IF_TY>), grd, blk, 0, s, B, H); break;
-        case 2: hipLaunchKernelGGL((k_iface<S, P, 2, IF_TX, IF_TY>), grd, blk, 0, s, B, H); break;
-        case 3: hipLaunchKernelGGL((k_iface<S, P, 3, IF_TX, IF_TY>), grd, blk, 0, s, B, H); break;
-        default: hipLaunchKernelGGL((k_iface<S, P, 4, IF_TX, IF_TY>), grd, blk, 0, s, B, H); break;
-        }
+        hipLaunchKernelGGL((k_iface<S, P, GR, IF_TX, IF_TY>), dim3(blocks), dim3(IF_TX * IF_TY), 0, s, B, H);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+template <class S, class P>
+static hipError_t iface_items(hipStream_t s, int GR, IfaceItem *items, int n, HermiteW<P> H) {
+    switch (GR) {
+    case 1: return iface_launch<S, P, 1>(s, items, n, H);
+    case 2: return iface_launch<S, P, 2>(s, items, n, H);
+    case 3: return iface_launch<S, P, 3>(s, items, n, H);
+    default: return iface_launch<S, P, 4>(s, items, n, H);
+    }
 }
 
 // both bands of one handle: lower writes its top stored layer, upper its layer 0
@@ -516,13 +548,7 @@ hipError_t launch_interface_many(hipStream_t s, dnsg_ctx *const *hs, int n, int 
     };
     H.exact = pow2_or_zero(H.h00) && pow2_or_zero(H.h10) && pow2_or_zero(H.h01) && pow2_or_zero(H.h11);
     std::vector<IfaceItem> v(2 * (size_t)n);
-    long tiles = 0;
-    for (int i = 0; i < n; i++) {
-        iface_pair(hs[i], &v[2 * (size_t)i]);
-        tiles += 2L * ((hs[i]->d.Xc + IF_TX - 1) / IF_TX) * ((hs[i]->d.Yc + IF_TY - 1) / IF_TY);
-    }
-    const int G = iface_groups(tiles);
-    for (auto &it : v) it.groups = G;
+    for (int i = 0; i < n; i++) iface_pair(hs[i], &v[2 * (size_t)i]);
     if (prec == DNSG_PREC_FP64) return iface_items<double, double>(s, GR, v.data(), 2 * n, H);
     HermiteW<float> Hf = {(float)H.h00, (float)H.h10, (float)H.h01, (float)H.h11, H.exact};
     if (prec == DNSG_PREC_FP32) return iface_items<float, float>(s, GR, v.data(), 2 * n, Hf);
