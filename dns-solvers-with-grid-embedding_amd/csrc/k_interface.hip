@@ -129,7 +129,8 @@ hipError_t launch_sample_many(hipStream_t s, dnsg_ctx *const *hs, int n) {
 template <class P>
 struct HermiteW {
     P h00, h10, h01, h11;
-    int exact;  // every weight is 0 or a power of two (gratio 1 and 2): products exact
+    int exact;  // every weight is 0 or a power of two, so its products are exact (every
+                // substep at gratio 1 and 2, substeps 2 and 4 at gratio 4)
 };
 
 // a b + c with one rounding, used only where the product a b is exact (a is 0 or a
