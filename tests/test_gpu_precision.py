@@ -51,6 +51,45 @@ def test_shifted_tier_filtered_injection(prec):
     assert relmax(dg["flowrate"], dr["flowrate"]) <= TOL[prec]
 
 
+@pytest.mark.parametrize("gratio", [1, 3])
+def test_shifted_tier_other_gratios(gratio):
+    """the FP32 bar at gratio 1 and 3: the shifted tiers' bicubic patch is a weight
+    matrix formed at compile time per gratio (fine points at thirds for gratio 3)"""
+    kw = dict(KW, gratio=gratio, zds=6)
+    n = 200
+    o = Oracle(dnsg.make_params(**kw), threads=4)
+    o.init_seeded()
+    dr = o.run(n)
+    s = dnsg.Solver(dnsg.make_params(**kw, precision="fp32"))
+    s.init_state()
+    dg = s.step(n)
+    sg, so = s.statistics(), o.statistics()
+    assert relmax(sg[0], so[0]) <= TOL["fp32"], relmax(sg[0], so[0])
+    assert relmax(dg["flowrate"], dr["flowrate"]) <= TOL["fp32"]
+
+
+@pytest.mark.parametrize("gratio", [1, 2, 3, 4])
+def test_fp32_state_after_two_steps(gratio):
+    """node by node after two coarse steps from the seeded state: FP32 arithmetic
+    leaves at most about 8e-9 on f (measured at gratio 1..4), while an interpolation
+    weight at the wrong fine point moves the interface layer by about 1e-4 (5% noise
+    on u = 0.05), so 1e-7 absolute separates the two; the band lattices include
+    their interface layer"""
+    kw = dict(KW, gratio=gratio, zds=6)
+    o = Oracle(dnsg.make_params(**kw), threads=4)
+    o.init_seeded()
+    o.run(2)
+    ref = o.state()
+    s = dnsg.Solver(dnsg.make_params(**kw, precision="fp32"))
+    s.init_state()
+    s.step(2)
+    g = s.download()
+    for k in ("coarse", "lower", "upper"):
+        err = float(np.max(np.abs(g[k] - ref[k])))
+        print(f"gratio {gratio} {k}: max |f32 - f64| = {err:.2e}", flush=True)
+        assert err <= 1e-7, (k, err)
+
+
 @pytest.mark.parametrize("prec", ["fp32", "fp16"])
 def test_shifted_state_roundtrip(prec):
     """upload of an FP64 state into shifted storage and back loses only the storage rounding"""
