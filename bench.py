@@ -65,10 +65,12 @@ def run_gpu(mod, prec, steps, warmup, force_mode="const"):
     s.init_state()
     if warmup:
         s.run_timed(warmup)
+    # the timed steps carry HIP events just before and after each band launch (on the
+    # solver's stream): the roofline's launch time comes from these same steps
     t0 = time.perf_counter()
-    ms_ev, diag = s.run_timed(steps)   # synchronises the stream before returning
+    ms_ev, diag, sec = s.run_timed(steps, sections="band")   # synchronises the stream
     wall = time.perf_counter() - t0
-    kt = s.kernel_times(3)
+    kt = s.kernel_times(3)   # every kernel group, back to back, in 3 further steps
     d = s.dims
     lu = lu_per_step(d)
     out = dict(glups=lu * steps / wall / 1e9, ms_per_step=wall * 1e3 / steps, ms_per_step_events=ms_ev / steps,
@@ -76,8 +78,9 @@ def run_gpu(mod, prec, steps, warmup, force_mode="const"):
                stress_last=float(diag["stress"][-1]))
     band_lu = 2 * d.Xf * d.Yf * (d.zdf - 1)       # both bands, one launch
     band_bytes = band_lu * BYTES_PER_LU[prec]
-    out["band_launch"] = dict(lu=band_lu, bytes=band_bytes, ms=kt["band"],
-                              gbs=band_bytes / (kt["band"] * 1e-3) / 1e9 if kt["band"] > 0 else None)
+    band_ms = sec["band"]
+    out["band_launch"] = dict(lu=band_lu, bytes=band_bytes, ms=band_ms, launches=steps * (d.GR - 1),
+                              gbs=band_bytes / (band_ms * 1e-3) / 1e9 if band_ms > 0 else None)
     s.close()
     return out
 

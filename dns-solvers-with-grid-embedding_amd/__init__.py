@@ -96,6 +96,7 @@ SIGNATURES = {
     "dnsg_download_state": [_P, _DP, _DP, _DP, _DP, _DP, _DP],
     "dnsg_step": [_P, ctypes.c_int, ctypes.POINTER(Diag)],
     "dnsg_run_timed": [_P, ctypes.c_int, ctypes.POINTER(Diag), _DP],
+    "dnsg_run_timed_sections": [_P, ctypes.c_int, ctypes.POINTER(Diag), _DP, _DP, ctypes.c_int],
     "dnsg_kernel_times": [_P, ctypes.c_int, _DP],
     "dnsg_hbm_probe": [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, _DP],
     "dnsg_encalc2": [_P, ctypes.c_long, ctypes.POINTER(Diag)],
@@ -123,6 +124,9 @@ SIGNATURES = {
 }
 
 _lib = None
+
+# kernel groups of dnsg_kernel_times / dnsg_run_timed_sections, in their order
+SECTIONS = ("reduce", "coarse", "sample", "band", "interface")
 
 
 def load(path=LIB_PATH):
@@ -261,17 +265,26 @@ class Solver:
         self._check(self.lib.dnsg_step(self.h, n, d))
         return diag_array(d, n)
 
-    def run_timed(self, n):
+    def run_timed(self, n, sections=None):
+        """n steps without per-step host synchronisation: (device ms, diag).  With
+        sections="all" (or one name of SECTIONS) also the average ms per launch of the
+        kernel groups (or of that one group) over those same steps, keys as
+        kernel_times"""
         d = (Diag * n)()
         ms = ctypes.c_double()
-        self._check(self.lib.dnsg_run_timed(self.h, n, d, ctypes.byref(ms)))
-        return ms.value, diag_array(d, n)
+        if sections is None:
+            self._check(self.lib.dnsg_run_timed(self.h, n, d, ctypes.byref(ms)))
+            return ms.value, diag_array(d, n)
+        group = -1 if sections == "all" else SECTIONS.index(sections)
+        sec = np.zeros(5)
+        self._check(self.lib.dnsg_run_timed_sections(self.h, n, d, ctypes.byref(ms), _ptr(sec), group))
+        return ms.value, diag_array(d, n), dict(zip(SECTIONS, sec))
 
     def kernel_times(self, n):
         """average ms per launch: reductions, coarse CS, sampling, band CS, interface"""
         ms = np.zeros(5)
         self._check(self.lib.dnsg_kernel_times(self.h, n, _ptr(ms)))
-        return dict(zip(("reduce", "coarse", "sample", "band", "interface"), ms))
+        return dict(zip(SECTIONS, ms))
 
     def mass(self):
         m = ctypes.c_double()

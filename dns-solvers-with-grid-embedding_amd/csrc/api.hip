@@ -218,17 +218,56 @@ int do_xfcomputations(dnsg_t *h) {
 // optional section timing: events recorded on the handle's stream between the
 // kernel groups of one step
 enum { SEC_REDUCE = 0, SEC_COARSE = 1, SEC_SAMPLE = 2, SEC_BAND = 3, SEC_IFACE = 4, SEC_N = 5 };
+// Every group (only < 0): one event at the start of the step and one after each
+// group, so the groups are timed back to back.  One group (only = k): an event
+// just before and just after each launch of group k, and no others.
 struct SecTimer {
     std::vector<hipEvent_t> ev;
-    std::vector<int> tag;  // section that ends at ev[i] (ev[0] starts the step)
-    void mark(hipStream_t s, int t) {
+    std::vector<int> tag;  // section that ends at ev[i]; -1 starts an interval
+    int only = -1;
+    explicit SecTimer(int group = -1) : only(group) {}
+    SecTimer(const SecTimer &) = delete;
+    SecTimer &operator=(const SecTimer &) = delete;
+    ~SecTimer() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    void record(hipStream_t s, int t) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return;
         (void)hipEventRecord(e, s);
         ev.push_back(e);
         tag.push_back(t);
     }
+    void start_step(hipStream_t s) {
+        if (only < 0) record(s, -1);
+    }
+    void begin(hipStream_t s, int t) {
+        if (t == only) record(s, -1);
+    }
+    void end(hipStream_t s, int t) {
+        if (only < 0 || t == only) record(s, t);
+    }
 };
+
+// average ms per launch (group) of each section over the recorded steps; the
+// stream must have passed the last event
+static int sec_average(dnsg_t *h, const SecTimer &T, double *ms) {
+    double acc[SEC_N] = {0., 0., 0., 0., 0.};
+    int cnt[SEC_N] = {0, 0, 0, 0, 0};
+    for (size_t i = 1; i < T.ev.size(); i++) {
+        if (T.tag[i] < 0) continue;
+        float t = 0.f;
+        const hipError_t e = hipEventElapsedTime(&t, T.ev[i - 1], T.ev[i]);
+        if (e != hipSuccess) {
+            h->err = std::string("event timing: ") + hipGetErrorString(e);
+            return DNSG_ERR_HIP;
+        }
+        acc[T.tag[i]] += t;
+        cnt[T.tag[i]]++;
+    }
+    for (int k = 0; k < SEC_N; k++) ms[k] = cnt[k] ? acc[k] / cnt[k] : 0.;
+    return DNSG_OK;
+}
 
 // One coarse step of main:1256-1375; rec receives the diag record on the device.
 // The reductions of encalc2 and wshearstr read the state at the top of the step,
@@ -246,10 +285,11 @@ int do_step(dnsg_t *h, double *rec, SecTimer *T = nullptr) {
     // or after the state was changed from outside)
     const bool from_writes = flux && h->flux_fused;
     const int GR = h->d.GR;
-    if (T) T->mark(h->stream, -1);
+    if (T) T->start_step(h->stream);
     if (flux && !from_writes) {
+        if (T) T->begin(h->stream, SEC_REDUCE);
         HIPCHK(h, launch_encalc2(h->stream, h, rec));
-        if (T) T->mark(h->stream, SEC_REDUCE);
+        if (T) T->end(h->stream, SEC_REDUCE);
     }
     double *dl = h->dsum + 2 * (size_t)h->d.Yc * h->d.zdc * cs_chunks(h->d.Xc);
     double *du = dl + 2 * (size_t)h->d.Yf * (h->d.zdf - 1) * cs_chunks(h->d.Xf);
@@ -273,25 +313,30 @@ int do_step(dnsg_t *h, double *rec, SecTimer *T = nullptr) {
             B[0].dsum = dl;
             B[1].dsum = du;
         }
+        if (T) T->begin(h->stream, iter == 1 ? SEC_COARSE : SEC_BAND);
         HIPCHK(h, launch_collide_stream(h->stream, L, nl, h->p.precision));
-        if (T) T->mark(h->stream, iter == 1 ? SEC_COARSE : SEC_BAND);
+        if (T) T->end(h->stream, iter == 1 ? SEC_COARSE : SEC_BAND);
         if (iter == 1) {
             rotate_levels(h);
+            if (T) T->begin(h->stream, SEC_SAMPLE);
             HIPCHK(h, launch_sample(h->stream, h, h->coarse.n(), h->lev_lo[0], h->lev_up[0]));
-            if (T) T->mark(h->stream, SEC_SAMPLE);
+            if (T) T->end(h->stream, SEC_SAMPLE);
+            if (T) T->begin(h->stream, SEC_REDUCE);
             if (!flux) HIPCHK(h, launch_encalc2_fused(h->stream, h, rec));
             HIPCHK(h, launch_wshear_fused(h->stream, h, rec, from_writes ? h->enc_next : nullptr));
-            if (T) T->mark(h->stream, SEC_REDUCE);
+            if (T) T->end(h->stream, SEC_REDUCE);
         }
+        if (T) T->begin(h->stream, SEC_IFACE);
         HIPCHK(h, launch_interface(h->stream, h, iter));
         if (iter == GR) HIPCHK(h, launch_ftcd(h->stream, h, flux ? h->fsum : nullptr));
-        if (T) T->mark(h->stream, SEC_IFACE);
+        if (T) T->end(h->stream, SEC_IFACE);
         h->lower.swap();
         h->upper.swap();
     }
     if (flux) {
+        if (T) T->begin(h->stream, SEC_REDUCE);
         HIPCHK(h, launch_flux_next(h->stream, h));
-        if (T) T->mark(h->stream, SEC_REDUCE);
+        if (T) T->end(h->stream, SEC_REDUCE);
     }
     h->flux_fused = flux;
     h->coarse.swap();
@@ -787,7 +832,11 @@ int dnsg_step(dnsg_t *h, int n, dnsg_diag *diag) {
 }
 
 int dnsg_run_timed(dnsg_t *h, int n, dnsg_diag *diag, double *elapsed_ms) {
-    if (!h || !h->stream || n < 1) return DNSG_ERR_ARG;
+    return dnsg_run_timed_sections(h, n, diag, elapsed_ms, nullptr, -1);
+}
+
+int dnsg_run_timed_sections(dnsg_t *h, int n, dnsg_diag *diag, double *elapsed_ms, double *sec_ms, int group) {
+    if (!h || !h->stream || n < 1 || group < -1 || group >= SEC_N) return DNSG_ERR_ARG;
     if (!h->have_state) {
         h->err = "no state: call dnsg_init_state or dnsg_upload_state first";
         return DNSG_ERR_STATE;
@@ -798,15 +847,19 @@ int dnsg_run_timed(dnsg_t *h, int n, dnsg_diag *diag, double *elapsed_ms) {
     HIPCHK(h, hipEventCreate(&e0));
     HIPCHK(h, hipEventCreate(&e1));
     const long ss0 = h->ss;
+    SecTimer T(group);  // section events of the timed steps, when sec_ms is given
     HIPCHK(h, hipEventRecord(e0, h->stream));
-    for (int s = 0; s < n; s++)
-        if ((rc = do_step(h, h->d_diag + (size_t)s * DIAG_WORDS))) return rc;
-    HIPCHK(h, hipEventRecord(e1, h->stream));
-    HIPCHK(h, hipEventSynchronize(e1));
+    for (int s = 0; s < n && !rc; s++) rc = do_step(h, h->d_diag + (size_t)s * DIAG_WORDS, sec_ms ? &T : nullptr);
+    if (!rc) {
+        HIPCHK(h, hipEventRecord(e1, h->stream));
+        HIPCHK(h, hipEventSynchronize(e1));
+    }
     float ms = 0.f;
-    HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+    if (!rc) HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (sec_ms && !rc) rc = sec_average(h, T, sec_ms);
+    if (rc) return rc;
     if (elapsed_ms) *elapsed_ms = ms;
     if ((rc = ensure_pinned(h, (size_t)n * DIAG_WORDS))) return rc;
     HIPCHK(h, hipMemcpyAsync(h->h_pinned, h->d_diag, sizeof(double) * DIAG_WORDS * n, hipMemcpyDeviceToHost,
@@ -831,30 +884,11 @@ int dnsg_kernel_times(dnsg_t *h, int n, double *ms) {
     }
     int rc;
     if ((rc = ensure_diag(h, 1))) return rc;
-    double acc[SEC_N] = {0., 0., 0., 0., 0.};
-    int cnt[SEC_N] = {0, 0, 0, 0, 0};
-    for (int s = 0; s < n; s++) {
-        SecTimer T;
-        rc = do_step(h, h->d_diag, &T);
-        if (!rc) {
-            hipError_t e = hipEventSynchronize(T.ev.back());
-            for (size_t i = 1; i < T.ev.size() && e == hipSuccess; i++) {
-                float t = 0.f;
-                e = hipEventElapsedTime(&t, T.ev[i - 1], T.ev[i]);
-                acc[T.tag[i]] += t;
-                cnt[T.tag[i]]++;
-            }
-            if (e != hipSuccess) {
-                h->err = std::string("event timing: ") + hipGetErrorString(e);
-                rc = DNSG_ERR_HIP;
-            }
-        }
-        for (hipEvent_t e : T.ev) (void)hipEventDestroy(e);
-        if (rc) return rc;
-    }
-    // average time per launch (group) of each section
-    for (int k = 0; k < SEC_N; k++) ms[k] = cnt[k] ? acc[k] / cnt[k] : 0.;
-    return DNSG_OK;
+    SecTimer T;
+    for (int s = 0; s < n && !rc; s++) rc = do_step(h, h->d_diag, &T);
+    if (!rc && !T.ev.empty()) HIPCHK(h, hipEventSynchronize(T.ev.back()));
+    if (!rc) rc = sec_average(h, T, ms);
+    return rc;
 }
 
 // ---- reference-granular entry points ------------------------------------------------

@@ -204,6 +204,15 @@ int dnsg_step(dnsg_t *h, int n, dnsg_diag *diag);
  * the n steps (device time, inputs resident in HBM). */
 int dnsg_run_timed(dnsg_t *h, int n, dnsg_diag *diag, double *elapsed_ms);
 
+/* dnsg_run_timed that also records HIP events in every timed step
+ * (instrumentation, no counterpart in the reference) and returns in sec_ms[5] the
+ * average device time per launch of the kernel groups of dnsg_kernel_times, in its
+ * order, over those same steps.  group = -1 times every group (an event after each);
+ * group = k in 0..4 times only group k (an event just before and after each of its
+ * launches, the others 0), which adds fewer events to the timed steps.
+ * sec_ms = NULL is dnsg_run_timed. */
+int dnsg_run_timed_sections(dnsg_t *h, int n, dnsg_diag *diag, double *elapsed_ms, double *sec_ms, int group);
+
 /* Instrumentation (no counterpart in the reference, which only times the whole
  * loop with MPI_Wtime, main:1252,1377): run n steps with HIP events between the
  * kernel groups on the handle's stream and return the average device time in ms
