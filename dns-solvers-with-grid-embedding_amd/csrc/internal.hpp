@@ -55,7 +55,7 @@ struct LatArgs {
 // Several independent work items in one launch: item i owns the workgroups
 // [first[i], first[i+1]) of a one-dimensional grid.  The table travels in the
 // kernel arguments, so a launch needs no device-side setup.
-constexpr int BATCH_MAX = 16;
+constexpr int BATCH_MAX = 30;
 template <class A>
 struct Batch {
     static_assert(sizeof(A) * BATCH_MAX + 4 * (BATCH_MAX + 2) <= 4096, "batch table exceeds the kernel-argument limit");
