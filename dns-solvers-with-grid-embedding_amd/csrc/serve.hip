@@ -5,8 +5,8 @@
 // solver handles on the queue's stream; each coarse step of the resident set is
 // one batched launch per kernel and precision (tables of up to BATCH_MAX items
 // per launch: lattices, bands or solves, by kernel), so small channels share the
-// GPU instead of each leaving most of it idle.  The per-solve arithmetic is the single-solve step's, unchanged: a
-// solve gives the same bits alone or in any mix.
+// GPU instead of each leaving most of it idle.  The per-solve arithmetic is the
+// single-solve step's, unchanged: a solve gives the same bits alone or in any mix.
 //
 // Convergence: every check_every steps the bulk velocity U = <rho u_x>/<rho>
 // (encalc2.c:109-131 weights) of every resident solve is copied back in one
