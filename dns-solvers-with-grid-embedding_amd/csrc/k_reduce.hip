@@ -186,7 +186,7 @@ __device__ __forceinline__ void enc_entry(const EncArgs &A, const EncSpace &S, i
     }
 }
 
-constexpr int ENC_SEG = 4096;  // entries per first-stage workgroup
+constexpr int ENC_SEG = 512;  // entries per first-stage workgroup (~240 workgroups at the Re_tau=180 size)
 
 __global__ __launch_bounds__(RB) void k_enc_seg(EncArgs A, double *seg) {
     __shared__ double lds[4][RB / 64];
