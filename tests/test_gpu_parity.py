@@ -152,6 +152,21 @@ def test_constant_force_bitwise(name):
     assert np.array_equal(dg["time"], dr["time"])
 
 
+def test_constant_force_bitwise_split_directions():
+    """a launch of 128 interface tiles (256 x 64 coarse columns, both bands): the FP64
+    fill shares each tile's 19 directions over about 6 workgroups (the small cases
+    above take one direction per workgroup, the full-size one all 19), bit for bit"""
+    kw = dict(xdv=256, ydv=64, zdv=12, gratio=2, zds=4, Rebs=20.0, ub=0.0333333333333, CFL=5.0,
+              pattern="rslip", gyplussize=8, wyplussize=8)
+    o, s = pair(kw, force_mode="const", g_const=2e-5)
+    n = 10
+    s.step(n)
+    o.run(n)
+    g, ref = s.download(), o.state()
+    for k in GRIDS + ("lin", "uin"):
+        assert np.array_equal(g[k], ref[k]), f"{k}: max diff {np.max(np.abs(g[k] - ref[k]))}"
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_flux_control_parity(name):
     o, s = pair(CASES[name])
