@@ -89,11 +89,14 @@ SIGNATURES = {
     "dnsg_create": [ctypes.POINTER(Params), ctypes.POINTER(_P)],
     "dnsg_destroy": [_P],
     "dnsg_get_dims": [_P, ctypes.POINTER(Dims)],
+    "dnsg_node_codes": [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     "dnsg_init_state": [_P],
     "dnsg_upload_state": [_P, _DP, _DP, _DP, _DP, _DP, ctypes.c_double],
     "dnsg_init_uniform": [_P, _DP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "dnsg_budget2d": [_P, _DP, _DP, _DP, _DP, _DP],
     "dnsg_download_state": [_P, _DP, _DP, _DP, _DP, _DP, _DP],
+    "dnsg_upload_state_ghosted": [_P, _DP, _DP, _DP, ctypes.POINTER(_DP), ctypes.POINTER(_DP), ctypes.c_double],
+    "dnsg_download_state_ghosted": [_P, _DP, _DP, _DP, ctypes.POINTER(_DP), ctypes.POINTER(_DP), _DP],
     "dnsg_step": [_P, ctypes.c_int, ctypes.POINTER(Diag)],
     "dnsg_run_timed": [_P, ctypes.c_int, ctypes.POINTER(Diag), _DP],
     "dnsg_run_timed_sections": [_P, ctypes.c_int, ctypes.POINTER(Diag), _DP, _DP, ctypes.c_int],
@@ -227,6 +230,16 @@ class Solver:
     def shapes(self):
         d = self.dims
         return dict(coarse=(d.Xc, d.Yc, d.zdc, 19), band=(d.Xf, d.Yf, d.zdf, 19), levels=(3, d.Xc, d.Yc, 23))
+
+    def node_codes(self):
+        """V.s [Xc+1][Yc+1][zdc], V.sls and V.sus [Xf+1][Yf+1][zdf] (int32 node codes)"""
+        d = self.dims
+        sc = np.empty((d.Xc + 1, d.Yc + 1, d.zdc), dtype=np.int32)
+        sl = np.empty((d.Xf + 1, d.Yf + 1, d.zdf), dtype=np.int32)
+        su = np.empty_like(sl)
+        P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        self._check(self.lib.dnsg_node_codes(self.h, P(sc), P(sl), P(su)))
+        return sc, sl, su
 
     def init_state(self):
         self._check(self.lib.dnsg_init_state(self.h))

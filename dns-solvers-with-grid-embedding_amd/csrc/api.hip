@@ -314,17 +314,19 @@ int do_step(dnsg_t *h, double *rec, SecTimer *T = nullptr) {
             B[1].dsum = du;
         }
         if (T) T->begin(h->stream, iter == 1 ? SEC_COARSE : SEC_BAND);
-        HIPCHK(h, launch_collide_stream(h->stream, L, nl, h->p.precision));
+        HIPCHK(h, launch_collide_stream(h->stream, L, nl, h->p.precision, h->halt()));
         if (T) T->end(h->stream, iter == 1 ? SEC_COARSE : SEC_BAND);
         if (iter == 1) {
-            rotate_levels(h);
-            if (T) T->begin(h->stream, SEC_SAMPLE);
-            HIPCHK(h, launch_sample(h->stream, h, h->coarse.n(), h->lev_lo[0], h->lev_up[0]));
-            if (T) T->end(h->stream, SEC_SAMPLE);
+            // the reductions first: a NaN wall stress halts every later launch, the
+            // sampling included, so the state stays that at the top of this step
             if (T) T->begin(h->stream, SEC_REDUCE);
             if (!flux) HIPCHK(h, launch_encalc2_fused(h->stream, h, rec));
             HIPCHK(h, launch_wshear_fused(h->stream, h, rec, from_writes ? h->enc_next : nullptr));
             if (T) T->end(h->stream, SEC_REDUCE);
+            rotate_levels(h);
+            if (T) T->begin(h->stream, SEC_SAMPLE);
+            HIPCHK(h, launch_sample(h->stream, h, h->coarse.n(), h->lev_lo[0], h->lev_up[0]));
+            if (T) T->end(h->stream, SEC_SAMPLE);
         }
         if (T) T->begin(h->stream, SEC_IFACE);
         HIPCHK(h, launch_interface(h->stream, h, iter));
@@ -356,20 +358,122 @@ void fill_diag(const dnsg_t *h, const double *rec, long ss, dnsg_diag *d) {
     d->pgrad = rec[8];
 }
 
-// [3][Xc][Yc][23] (file layout, newest first) <-> three device planes [23][Yc][Xc];
-// the shifted tiers keep FP32 planes with rho - 1 in slot 0
-int put_levels(dnsg_t *h, const double *lv, void *const *dev) {
+// The pointer state at the top of a step: which buffer of each grid is current,
+// the interface levels, and whether the flux controller's G for it is formed.
+struct StepTop {
+    int cur[3];
+    void *lo[3], *up[3];
+    bool flux_fused;
+};
+
+StepTop step_top(const dnsg_t *h) {
+    StepTop t;
+    t.cur[0] = h->coarse.cur; t.cur[1] = h->lower.cur; t.cur[2] = h->upper.cur;
+    for (int l = 0; l < 3; l++) { t.lo[l] = h->lev_lo[l]; t.up[l] = h->lev_up[l]; }
+    t.flux_fused = h->flux_fused;
+    return t;
+}
+
+void restore_top(dnsg_t *h, const StepTop &t) {
+    h->coarse.cur = t.cur[0]; h->lower.cur = t.cur[1]; h->upper.cur = t.cur[2];
+    for (int l = 0; l < 3; l++) { h->lev_lo[l] = t.lo[l]; h->lev_up[l] = t.up[l]; }
+    h->flux_fused = t.flux_fused;
+}
+
+// Queue m steps without host synchronisation, then copy the m diag records back.
+// A NaN wall stress in step s (wshearstr.c:139-143 aborts there) sets the device
+// halt flag, after which every state-changing launch of the queued steps returns
+// at once; the host then rewinds its buffer and level pointers to the top of step
+// s, so the state, ss and force are those the reference holds when it aborts, and
+// the records up to and including step s are filled.  With ms, the HIP-event time
+// of the m steps; with T, section events inside them.
+int run_steps(dnsg_t *h, int m, dnsg_diag *diag, float *ms, SecTimer *T) {
+    int rc;
+    if ((rc = ensure_diag(h, m))) return rc;
+    if ((rc = ensure_pinned(h, (size_t)m * DIAG_WORDS))) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ms) {
+        HIPCHK(h, hipEventCreate(&e0));
+        if (hipEventCreate(&e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            h->err = "hipEventCreate failed";
+            return DNSG_ERR_HIP;
+        }
+    }
+    std::vector<StepTop> tops;
+    tops.reserve(m);
+    const long ss0 = h->ss;
+    hipError_t e = hipMemsetAsync(h->d_halt, 0, sizeof(int), h->stream);
+    if (e == hipSuccess && ms) e = hipEventRecord(e0, h->stream);
+    rc = e == hipSuccess ? DNSG_OK : DNSG_ERR_HIP;
+    if (rc) h->err = std::string("step set-up: ") + hipGetErrorString(e);
+    h->halt_armed = true;
+    for (int s = 0; s < m && !rc; s++) {
+        tops.push_back(step_top(h));
+        rc = do_step(h, h->d_diag + (size_t)s * DIAG_WORDS, T);
+    }
+    h->halt_armed = false;
+    if (!rc && ms) {
+        e = hipEventRecord(e1, h->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+        if (e != hipSuccess) {
+            h->err = std::string("step timing: ") + hipGetErrorString(e);
+            rc = DNSG_ERR_HIP;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->h_pinned, h->d_diag, sizeof(double) * DIAG_WORDS * m, hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int s = 0; s < m; s++) {
+        const double *rec = h->h_pinned + (size_t)s * DIAG_WORDS;
+        if (diag) fill_diag(h, rec, ss0 + s, diag + s);
+        if (isnan(rec[7])) {
+            restore_top(h, tops[s]);
+            h->ss = ss0 + s;
+            h->force = rec[1];
+            char buf[96];
+            snprintf(buf, sizeof buf, "NaN in wall shear stress at step %ld", ss0 + s);
+            h->err = buf;
+            return DNSG_ERR_NAN;
+        }
+    }
+    h->force = h->h_pinned[(size_t)(m - 1) * DIAG_WORDS + 1];
+    return DNSG_OK;
+}
+
+// Host layouts of the state.  File layout (ghost = false): lattices [X][Y][Z][19],
+// interface levels [Xc][Yc][23].  The reference's in-memory layout (ghost = true,
+// main:145-151, 473-883): lattices [X+1][Y+1][Z][19] whose last x-plane and y-row
+// are ghost cells (the halo of the periodic neighbour, main:181-189), interface
+// levels [Xc+6][Yc+6][23] with the interior at offset (2, 2) (Fbi,
+// definitions.h:132; V.lin allocation main:827-833).  Uploads read the interior;
+// downloads also write every ghost cell as its periodic image.
+struct HostLayout {
+    bool ghost = false;
+    int gl() const { return ghost ? 1 : 0; }              // extra lattice planes in x and y
+    size_t lev_at(int i, int j, int v, int Xc, int Yc) const {  // interior (i, j) of a level
+        return ghost ? ((size_t)(i + 2) * (Yc + 6) + (j + 2)) * 23 + v : ((size_t)i * Yc + j) * 23 + v;
+    }
+};
+
+// three host levels (newest first) <-> three device planes [23][Yc][Xc]; the shifted
+// tiers keep FP32 planes with rho - 1 in slot 0
+int put_levels(dnsg_t *h, const double *const *lv, void *const *dev, HostLayout L = {}) {
     const int Xc = h->d.Xc, Yc = h->d.Yc;
     const size_t np = (size_t)Xc * Yc * NVP;
     const bool f32 = h->p.precision != DNSG_PREC_FP64;
     std::vector<double> t(np);
     std::vector<float> tf(f32 ? np : 0);
     for (int l = 0; l < 3; l++) {
-        const double *src = lv + l * np;
+        const double *src = lv[l];
         for (int i = 0; i < Xc; i++)
             for (int j = 0; j < Yc; j++)
                 for (int v = 0; v < NVP; v++) {
-                    const double x = src[((size_t)i * Yc + j) * NVP + v];
+                    const double x = src[L.lev_at(i, j, v, Xc, Yc)];
                     const size_t d = ((size_t)v * Yc + j) * Xc + i;
                     if (f32) tf[d] = (float)(v == 0 ? x - 1. : x);
                     else t[d] = x;
@@ -381,7 +485,7 @@ int put_levels(dnsg_t *h, const double *lv, void *const *dev) {
     return DNSG_OK;
 }
 
-int get_levels(dnsg_t *h, double *lv, void *const *dev) {
+int get_levels(dnsg_t *h, double *const *lv, void *const *dev, HostLayout L = {}) {
     const int Xc = h->d.Xc, Yc = h->d.Yc;
     const size_t np = (size_t)Xc * Yc * NVP;
     const bool f32 = h->p.precision != DNSG_PREC_FP64;
@@ -391,36 +495,45 @@ int get_levels(dnsg_t *h, double *lv, void *const *dev) {
         if (f32) HIPCHK(h, hipMemcpyAsync(tf.data(), dev[l], np * 4, hipMemcpyDeviceToHost, h->stream));
         else HIPCHK(h, hipMemcpyAsync(t.data(), dev[l], np * 8, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
-        double *dst = lv + l * np;
-        for (int i = 0; i < Xc; i++)
-            for (int j = 0; j < Yc; j++)
+        double *dst = lv[l];
+        const int g = L.ghost ? 2 : 0, ge = L.ghost ? 4 : 0;  // halo cells before / after the interior
+        for (int i = -g; i < Xc + ge; i++)
+            for (int j = -g; j < Yc + ge; j++) {
+                const int ii = (i + Xc) % Xc, jj = (j + Yc) % Yc;  // periodic image
                 for (int v = 0; v < NVP; v++) {
-                    const size_t d = ((size_t)v * Yc + j) * Xc + i;
-                    dst[((size_t)i * Yc + j) * NVP + v] = f32 ? (v == 0 ? 1. + (double)tf[d] : (double)tf[d]) : t[d];
+                    const size_t d = ((size_t)v * Yc + jj) * Xc + ii;
+                    const size_t at = L.ghost ? ((size_t)(i + 2) * (Yc + 6) + (j + 2)) * 23 + v
+                                              : ((size_t)i * Yc + j) * 23 + v;
+                    dst[at] = f32 ? (v == 0 ? 1. + (double)tf[d] : (double)tf[d]) : t[d];
                 }
+            }
     }
     return DNSG_OK;
 }
 
-// AoS [X][Y][zaos][19] host array <-> SoA device lattice (stored layers z0..z0+nz-1)
-int put_lattice(dnsg_t *h, const double *aos, Grid &g, int zaos, int z0) {
-    const size_t n = (size_t)g.nx * g.ny * zaos * 19;
+// AoS [X(+1)][Y(+1)][zaos][19] host array <-> SoA device lattice (stored layers z0..z0+nz-1)
+int put_lattice(dnsg_t *h, const double *aos, Grid &g, int zaos, int z0, HostLayout L = {}) {
+    const int gl = L.gl();
+    const size_t n = (size_t)(g.nx + gl) * (g.ny + gl) * zaos * 19;
     double *stage = nullptr;
     HIPCHK(h, hipMalloc(&stage, n * 8));
     hipError_t e = hipMemcpyAsync(stage, aos, n * 8, hipMemcpyHostToDevice, h->stream);
-    if (e == hipSuccess) e = launch_aos_to_soa(h->stream, stage, g.c(), g.nx, g.ny, zaos, z0, g.nz, h->p.precision);
+    if (e == hipSuccess)
+        e = launch_aos_to_soa(h->stream, stage, g.c(), g.nx, g.ny, zaos, z0, g.nz, h->p.precision, gl);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     (void)hipFree(stage);
     HIPCHK(h, e);
     return DNSG_OK;
 }
 
-int get_lattice(dnsg_t *h, double *aos, const Grid &g, int zaos, int z0) {
-    const size_t n = (size_t)g.nx * g.ny * zaos * 19;
+int get_lattice(dnsg_t *h, double *aos, const Grid &g, int zaos, int z0, HostLayout L = {}) {
+    const int gl = L.gl();
+    const size_t n = (size_t)(g.nx + gl) * (g.ny + gl) * zaos * 19;
     double *stage = nullptr;
     HIPCHK(h, hipMalloc(&stage, n * 8));
     hipError_t e = hipMemsetAsync(stage, 0, n * 8, h->stream);
-    if (e == hipSuccess) e = launch_soa_to_aos(h->stream, g.c(), stage, g.nx, g.ny, zaos, z0, g.nz, h->p.precision);
+    if (e == hipSuccess)
+        e = launch_soa_to_aos(h->stream, g.c(), stage, g.nx, g.ny, zaos, z0, g.nz, h->p.precision, gl);
     if (e == hipSuccess) e = hipMemcpyAsync(aos, stage, n * 8, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     (void)hipFree(stage);
@@ -429,12 +542,12 @@ int get_lattice(dnsg_t *h, double *aos, const Grid &g, int zaos, int z0) {
 }
 
 // keep the wall layer k of a band AoS array on the host; returns its mass
-double keep_wall(const dnsg_t *h, const double *aos, int k, double *dst) {
-    const int Xf = h->d.Xf, Yf = h->d.Yf, zdf = h->d.zdf;
+double keep_wall(const dnsg_t *h, const double *aos, int k, double *dst, HostLayout L = {}) {
+    const int Xf = h->d.Xf, Yf = h->d.Yf, zdf = h->d.zdf, ldy = Yf + L.gl();
     double m = 0.;
     for (int i = 0; i < Xf; i++)
         for (int j = 0; j < Yf; j++) {
-            const double *src = aos + (((size_t)i * Yf + j) * zdf + k) * 19;
+            const double *src = aos + (((size_t)i * ldy + j) * zdf + k) * 19;
             double r = 0.;
             for (int a = 0; a < 19; a++) {
                 dst[((size_t)i * Yf + j) * 19 + a] = src[a];
@@ -445,11 +558,11 @@ double keep_wall(const dnsg_t *h, const double *aos, int k, double *dst) {
     return m;
 }
 
-void place_wall(const dnsg_t *h, double *aos, int k, const double *src) {
-    const int Xf = h->d.Xf, Yf = h->d.Yf, zdf = h->d.zdf;
-    for (int i = 0; i < Xf; i++)
-        for (int j = 0; j < Yf; j++)
-            memcpy(aos + (((size_t)i * Yf + j) * zdf + k) * 19, src + ((size_t)i * Yf + j) * 19, 19 * 8);
+void place_wall(const dnsg_t *h, double *aos, int k, const double *src, HostLayout L = {}) {
+    const int Xf = h->d.Xf, Yf = h->d.Yf, zdf = h->d.zdf, gl = L.gl(), ldy = Yf + gl;
+    for (int i = 0; i < Xf + gl; i++)
+        for (int j = 0; j < Yf + gl; j++)
+            memcpy(aos + (((size_t)i * ldy + j) * zdf + k) * 19, src + ((size_t)(i % Xf) * Yf + (j % Yf)) * 19, 19 * 8);
 }
 
 }  // namespace
@@ -500,6 +613,59 @@ int dnsg_get_dims(const dnsg_t *h, dnsg_dims *d) {
     return DNSG_OK;
 }
 
+// The node codes as the reference holds them (solid_init.c:14-19: -1 no-slip wall,
+// -2 slip wall, 0 fluid, 1 periodic / bounded), read from what the kernels of this
+// handle use: the wall codes from the device wall maps, the bounded layers from the
+// collide-stream's boundary set-up (the layers whose outgoing populations are dropped
+// and rewritten by an interface transfer), and 1 on the periodic x/y edge planes of
+// the one-rank ghosted extents.
+int dnsg_node_codes(dnsg_t *h, int *s_c, int *s_l, int *s_u) {
+    if (!h || !h->stream) return DNSG_ERR_ARG;
+    const int Xc = h->d.Xc, Yc = h->d.Yc, Xf = h->d.Xf, Yf = h->d.Yf, zdc = h->d.zdc, zdf = h->d.zdf;
+    std::vector<int8_t> wl((size_t)Xf * Yf), wu((size_t)Xf * Yf);
+    LatArgs C = coarse_args(h), B[2];
+    band_args(h, B);
+    if (B[0].lo_kind != BOUND_WALL || B[1].hi_kind != BOUND_WALL || B[0].wlo == nullptr || B[1].whi == nullptr) {
+        h->err = "band wall boundaries not set up";
+        return DNSG_ERR_STATE;
+    }
+    HIPCHK(h, hipMemcpyAsync(wl.data(), B[0].wlo, wl.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(wu.data(), B[1].whi, wu.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const int xdc = Xc + 1, ydc = Yc + 1, xdf = Xf + 1, ydf = Yf + 1;
+    if (s_c)
+        for (int i = 0; i < xdc; i++)
+            for (int j = 0; j < ydc; j++)
+                for (int k = 0; k < zdc; k++) {
+                    const bool edge = i == 0 || i == xdc - 1 || j == 0 || j == ydc - 1;
+                    const bool bounded = (k == 0 && C.skip_lo) || (k == zdc - 1 && C.skip_hi);
+                    s_c[((size_t)i * ydc + j) * zdc + k] = (edge || bounded) ? 1 : 0;
+                }
+    // lower band: stored layer m = k - 1 (k = 0 is the wall); upper band: m = k
+    for (int b = 0; b < 2; b++) {
+        int *s = b == 0 ? s_l : s_u;
+        if (!s) continue;
+        const LatArgs &L = B[b];
+        const std::vector<int8_t> &w = b == 0 ? wl : wu;
+        const int kwall = b == 0 ? 0 : zdf - 1, kofs = b == 0 ? 1 : 0;
+        for (int i = 0; i < xdf; i++)
+            for (int j = 0; j < ydf; j++)
+                for (int k = 0; k < zdf; k++) {
+                    int c;
+                    if (k == kwall) {
+                        c = w[(size_t)(j % Yf) * Xf + (i % Xf)];  // the ghost planes hold the periodic image
+                    } else {
+                        const int m = k - kofs;
+                        const bool edge = i == 0 || i == xdf - 1 || j == 0 || j == ydf - 1;
+                        const bool bounded = (m == 0 && L.skip_lo) || (m == L.nz - 1 && L.skip_hi);
+                        c = (edge || bounded) ? 1 : 0;
+                    }
+                    s[((size_t)i * ydf + j) * zdf + k] = c;
+                }
+    }
+    return DNSG_OK;
+}
+
 int dnsg_destroy(dnsg_t *h) {
     if (!h) return DNSG_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
@@ -512,7 +678,7 @@ int dnsg_destroy(dnsg_t *h) {
         if (h->lev_up[l]) (void)hipFree(h->lev_up[l]);
     }
     void *ptrs[] = {h->wall_lo, h->wall_up, h->part, h->red, h->seg, h->dsum, h->fsum, h->enc_next,
-                    h->d_force, h->d_diag, h->d_red, h->d_acc, h->fwall, h->fneq};
+                    h->d_force, h->d_halt, h->d_diag, h->d_red, h->d_acc, h->fwall, h->fneq};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (h->h_pinned) (void)hipHostFree(h->h_pinned);
@@ -585,6 +751,8 @@ int dnsg::create_ctx(const dnsg_params *p, hipStream_t ext, dnsg_t **out) {
     }
     HIPCHK(h, hipMalloc(&h->d_force, 4 * 8));
     HIPCHK(h, hipMemsetAsync(h->d_force, 0, 4 * 8, h->stream));
+    HIPCHK(h, hipMalloc(&h->d_halt, sizeof(int)));
+    HIPCHK(h, hipMemsetAsync(h->d_halt, 0, sizeof(int), h->stream));
     const size_t nred = 4 * ((size_t)d.zdc + 2 * (d.zdf - 1)) + 2 * 9 * (size_t)d.NZ + 64;
     HIPCHK(h, hipMalloc(&h->d_red, nred * 8));
     HIPCHK(h, hipMemsetAsync(h->d_red, 0, nred * 8, h->stream));
@@ -683,36 +851,54 @@ int dnsg_init_state(dnsg_t *h) {
     return set_force(h, h->p.force_mode == DNSG_FORCE_CONST ? h->p.g_const : 0.);
 }
 
-int dnsg_upload_state(dnsg_t *h, const double *coarse, const double *lower, const double *upper, const double *lin3,
-                      const double *uin3, double force) {
+// the state of the '-p' read path, in either host layout (lin/uin: newest first)
+static int upload_any(dnsg_t *h, const double *coarse, const double *lower, const double *upper,
+                      const double *const *lin, const double *const *uin, double force, HostLayout L) {
     if (h) h->flux_fused = false;
     if (!h || !h->stream) return DNSG_ERR_ARG;
     int rc;
     const int zdf = h->d.zdf;
-    if (coarse && (rc = put_lattice(h, coarse, h->coarse, h->d.zdc, 0))) return rc;
+    if (coarse && (rc = put_lattice(h, coarse, h->coarse, h->d.zdc, 0, L))) return rc;
     double wl = 0., wu = 0.;
     bool wall_changed = false;
     if (lower) {
-        if ((rc = put_lattice(h, lower, h->lower, zdf, 1))) return rc;  // stored m = k-1
-        wl = keep_wall(h, lower, 0, h->host_wall_lo);
+        if ((rc = put_lattice(h, lower, h->lower, zdf, 1, L))) return rc;  // stored m = k-1
+        wl = keep_wall(h, lower, 0, h->host_wall_lo, L);
         wall_changed = true;
     } else {
         for (size_t n = 0; n < (size_t)h->d.Xf * h->d.Yf * 19; n++) wl += h->host_wall_lo[n];
     }
     if (upper) {
-        if ((rc = put_lattice(h, upper, h->upper, zdf, 0))) return rc;  // stored m = k
-        wu = keep_wall(h, upper, zdf - 1, h->host_wall_up);
+        if ((rc = put_lattice(h, upper, h->upper, zdf, 0, L))) return rc;  // stored m = k
+        wu = keep_wall(h, upper, zdf - 1, h->host_wall_up, L);
         wall_changed = true;
     } else {
         for (size_t n = 0; n < (size_t)h->d.Xf * h->d.Yf * 19; n++) wu += h->host_wall_up[n];
     }
     if (wall_changed) h->wall_mass = wl + wu;
-    if (lin3 && (rc = put_levels(h, lin3, h->lev_lo))) return rc;
-    if (uin3 && (rc = put_levels(h, uin3, h->lev_up))) return rc;
+    if (lin && (rc = put_levels(h, lin, h->lev_lo, L))) return rc;
+    if (uin && (rc = put_levels(h, uin, h->lev_up, L))) return rc;
     h->ss = 0;
     h->have_state = true;
     if ((rc = sync_fwall(h))) return rc;
     return set_force(h, h->p.force_mode == DNSG_FORCE_CONST ? h->p.g_const : force);
+}
+
+int dnsg_upload_state(dnsg_t *h, const double *coarse, const double *lower, const double *upper, const double *lin3,
+                      const double *uin3, double force) {
+    if (!h) return DNSG_ERR_ARG;
+    const size_t np = (size_t)h->d.Xc * h->d.Yc * NVP;
+    const double *lin[3] = {lin3, lin3 + np, lin3 + 2 * np}, *uin[3] = {uin3, uin3 + np, uin3 + 2 * np};
+    return upload_any(h, coarse, lower, upper, lin3 ? lin : nullptr, uin3 ? uin : nullptr, force, HostLayout{});
+}
+
+int dnsg_upload_state_ghosted(dnsg_t *h, const double *ftemp, const double *slftemp, const double *suftemp,
+                              const double *const *lin, const double *const *uin, double force) {
+    if (lin && (!lin[0] || !lin[1] || !lin[2])) return DNSG_ERR_ARG;
+    if (uin && (!uin[0] || !uin[1] || !uin[2])) return DNSG_ERR_ARG;
+    HostLayout L;
+    L.ghost = true;
+    return upload_any(h, ftemp, slftemp, suftemp, lin, uin, force, L);
 }
 
 int dnsg_init_uniform(dnsg_t *h, const double *uniform, int xdu, int ydu, int zdu) {
@@ -756,7 +942,9 @@ int dnsg_init_uniform(dnsg_t *h, const double *uniform, int xdu, int ydu, int zd
     const int Xc = h->d.Xc, Yc = h->d.Yc;
     const size_t np = (size_t)Xc * Yc * NVP;
     std::vector<double> lo(3 * np), up(3 * np);
-    if ((rc = get_levels(h, lo.data(), h->lev_lo)) || (rc = get_levels(h, up.data(), h->lev_up))) return rc;
+    double *lop[3] = {lo.data(), lo.data() + np, lo.data() + 2 * np};
+    double *upp[3] = {up.data(), up.data() + np, up.data() + 2 * np};
+    if ((rc = get_levels(h, lop, h->lev_lo)) || (rc = get_levels(h, upp, h->lev_up))) return rc;
     const size_t lim = (size_t)19 * (Xc + 6) * (Yc + 6);
     for (int i = 0; i < Xc; i++)
         for (int j = 0; j < Yc; j++)
@@ -770,28 +958,28 @@ int dnsg_init_uniform(dnsg_t *h, const double *uniform, int xdu, int ydu, int zd
                 lo[np + at] = lo[2 * np + at] = copied ? lo[at] : 0.;
                 up[np + at] = up[2 * np + at] = copied ? up[at] : 0.;
             }
-    if ((rc = put_levels(h, lo.data(), h->lev_lo)) || (rc = put_levels(h, up.data(), h->lev_up))) return rc;
+    if ((rc = put_levels(h, lop, h->lev_lo)) || (rc = put_levels(h, upp, h->lev_up))) return rc;
     h->ss = 0;
     h->have_state = true;
     return set_force(h, h->p.force_mode == DNSG_FORCE_CONST ? h->p.g_const : 0.);
 }
 
-int dnsg_download_state(dnsg_t *h, double *coarse, double *lower, double *upper, double *lin3, double *uin3,
-                        double *force) {
+static int download_any(dnsg_t *h, double *coarse, double *lower, double *upper, double *const *lin,
+                        double *const *uin, double *force, HostLayout L) {
     if (!h || !h->stream) return DNSG_ERR_ARG;
     int rc;
     const int zdf = h->d.zdf;
-    if (coarse && (rc = get_lattice(h, coarse, h->coarse, h->d.zdc, 0))) return rc;
+    if (coarse && (rc = get_lattice(h, coarse, h->coarse, h->d.zdc, 0, L))) return rc;
     if (lower) {
-        if ((rc = get_lattice(h, lower, h->lower, zdf, 1))) return rc;
-        place_wall(h, lower, 0, h->host_wall_lo);
+        if ((rc = get_lattice(h, lower, h->lower, zdf, 1, L))) return rc;
+        place_wall(h, lower, 0, h->host_wall_lo, L);
     }
     if (upper) {
-        if ((rc = get_lattice(h, upper, h->upper, zdf, 0))) return rc;
-        place_wall(h, upper, zdf - 1, h->host_wall_up);
+        if ((rc = get_lattice(h, upper, h->upper, zdf, 0, L))) return rc;
+        place_wall(h, upper, zdf - 1, h->host_wall_up, L);
     }
-    if (lin3 && (rc = get_levels(h, lin3, h->lev_lo))) return rc;
-    if (uin3 && (rc = get_levels(h, uin3, h->lev_up))) return rc;
+    if (lin && (rc = get_levels(h, lin, h->lev_lo, L))) return rc;
+    if (uin && (rc = get_levels(h, uin, h->lev_up, L))) return rc;
     if (force) {
         HIPCHK(h, hipMemcpyAsync(h->h_pinned, h->d_force, 8, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -800,33 +988,33 @@ int dnsg_download_state(dnsg_t *h, double *coarse, double *lower, double *upper,
     return DNSG_OK;
 }
 
+int dnsg_download_state(dnsg_t *h, double *coarse, double *lower, double *upper, double *lin3, double *uin3,
+                        double *force) {
+    if (!h) return DNSG_ERR_ARG;
+    const size_t np = (size_t)h->d.Xc * h->d.Yc * NVP;
+    double *lin[3] = {lin3, lin3 + np, lin3 + 2 * np}, *uin[3] = {uin3, uin3 + np, uin3 + 2 * np};
+    return download_any(h, coarse, lower, upper, lin3 ? lin : nullptr, uin3 ? uin : nullptr, force, HostLayout{});
+}
+
+int dnsg_download_state_ghosted(dnsg_t *h, double *ftemp, double *slftemp, double *suftemp, double *const *lin,
+                                double *const *uin, double *force) {
+    if (lin && (!lin[0] || !lin[1] || !lin[2])) return DNSG_ERR_ARG;
+    if (uin && (!uin[0] || !uin[1] || !uin[2])) return DNSG_ERR_ARG;
+    HostLayout L;
+    L.ghost = true;
+    return download_any(h, ftemp, slftemp, suftemp, lin, uin, force, L);
+}
+
 int dnsg_step(dnsg_t *h, int n, dnsg_diag *diag) {
     if (!h || !h->stream || n < 0) return DNSG_ERR_ARG;
     if (!h->have_state) {
         h->err = "no state: call dnsg_init_state or dnsg_upload_state first";
         return DNSG_ERR_STATE;
     }
-    int rc;
     const int chunk = 256;
     for (int s0 = 0; s0 < n; s0 += chunk) {
         const int m = (n - s0 < chunk) ? n - s0 : chunk;
-        if ((rc = ensure_diag(h, m))) return rc;
-        const long ss0 = h->ss;
-        for (int s = 0; s < m; s++)
-            if ((rc = do_step(h, h->d_diag + (size_t)s * DIAG_WORDS))) return rc;
-        if ((rc = ensure_pinned(h, (size_t)m * DIAG_WORDS))) return rc;
-        HIPCHK(h, hipMemcpyAsync(h->h_pinned, h->d_diag, sizeof(double) * DIAG_WORDS * m, hipMemcpyDeviceToHost,
-                                 h->stream));
-        HIPCHK(h, hipStreamSynchronize(h->stream));
-        for (int s = 0; s < m; s++) {
-            const double *rec = h->h_pinned + (size_t)s * DIAG_WORDS;
-            if (diag) fill_diag(h, rec, ss0 + s, diag + s0 + s);
-            if (isnan(rec[7])) {
-                h->err = "NaN in wall shear stress";
-                return DNSG_ERR_NAN;
-            }
-        }
-        h->force = h->h_pinned[(size_t)(m - 1) * DIAG_WORDS + 1];
+        if (int rc = run_steps(h, m, diag ? diag + s0 : nullptr, nullptr, nullptr)) return rc;
     }
     return DNSG_OK;
 }
@@ -841,38 +1029,12 @@ int dnsg_run_timed_sections(dnsg_t *h, int n, dnsg_diag *diag, double *elapsed_m
         h->err = "no state: call dnsg_init_state or dnsg_upload_state first";
         return DNSG_ERR_STATE;
     }
-    int rc;
-    if ((rc = ensure_diag(h, n))) return rc;
-    hipEvent_t e0, e1;
-    HIPCHK(h, hipEventCreate(&e0));
-    HIPCHK(h, hipEventCreate(&e1));
-    const long ss0 = h->ss;
     SecTimer T(group);  // section events of the timed steps, when sec_ms is given
-    HIPCHK(h, hipEventRecord(e0, h->stream));
-    for (int s = 0; s < n && !rc; s++) rc = do_step(h, h->d_diag + (size_t)s * DIAG_WORDS, sec_ms ? &T : nullptr);
-    if (!rc) {
-        HIPCHK(h, hipEventRecord(e1, h->stream));
-        HIPCHK(h, hipEventSynchronize(e1));
-    }
     float ms = 0.f;
-    if (!rc) HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (sec_ms && !rc) rc = sec_average(h, T, sec_ms);
+    int rc = run_steps(h, n, diag, &ms, sec_ms ? &T : nullptr);
     if (rc) return rc;
+    if (sec_ms && (rc = sec_average(h, T, sec_ms))) return rc;
     if (elapsed_ms) *elapsed_ms = ms;
-    if ((rc = ensure_pinned(h, (size_t)n * DIAG_WORDS))) return rc;
-    HIPCHK(h, hipMemcpyAsync(h->h_pinned, h->d_diag, sizeof(double) * DIAG_WORDS * n, hipMemcpyDeviceToHost,
-                             h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    for (int s = 0; s < n; s++) {
-        const double *rec = h->h_pinned + (size_t)s * DIAG_WORDS;
-        if (diag) fill_diag(h, rec, ss0 + s, diag + s);
-        if (isnan(rec[7])) {
-            h->err = "NaN in wall shear stress";
-            return DNSG_ERR_NAN;
-        }
-    }
     return DNSG_OK;
 }
 
@@ -893,6 +1055,7 @@ int dnsg_kernel_times(dnsg_t *h, int n, double *ms) {
 
 // ---- reference-granular entry points ------------------------------------------------
 
+// copy the one diag record of a granular call back; DNSG_OK or a HIP error
 static int scalar_call(dnsg_t *h, long ss, dnsg_diag *d) {
     HIPCHK(h, hipMemcpyAsync(h->h_pinned, h->d_diag, sizeof(double) * DIAG_WORDS, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -900,9 +1063,8 @@ static int scalar_call(dnsg_t *h, long ss, dnsg_diag *d) {
         dnsg_diag t;
         fill_diag(h, h->h_pinned, ss, &t);
         d->time = t.time;
-        return 1;
     }
-    return 0;
+    return DNSG_OK;
 }
 
 int dnsg_encalc2(dnsg_t *h, long ss, dnsg_diag *d) {
@@ -911,7 +1073,7 @@ int dnsg_encalc2(dnsg_t *h, long ss, dnsg_diag *d) {
         HIPCHK(h, hipMemcpyAsync(h->d_diag + 1, h->enc_next, 4 * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     else
         HIPCHK(h, launch_encalc2(h->stream, h, h->d_diag));
-    scalar_call(h, ss, d);
+    if (int rc = scalar_call(h, ss, d)) return rc;
     const double *rec = h->h_pinned;
     h->force = rec[1];
     if (d) { d->force = rec[1]; d->momentum = rec[2]; d->density = rec[3]; d->flowrate = rec[4]; }
@@ -921,7 +1083,7 @@ int dnsg_encalc2(dnsg_t *h, long ss, dnsg_diag *d) {
 int dnsg_wshearstr(dnsg_t *h, long ss, dnsg_diag *d) {
     if (!h || !h->stream) return DNSG_ERR_ARG;
     HIPCHK(h, launch_wshear(h->stream, h, h->d_diag));
-    scalar_call(h, ss, d);
+    if (int rc = scalar_call(h, ss, d)) return rc;
     const double *rec = h->h_pinned;
     if (d) { d->str_up = rec[5]; d->str_down = rec[6]; d->stress = rec[7]; d->pgrad = rec[8]; }
     if (isnan(rec[7])) {

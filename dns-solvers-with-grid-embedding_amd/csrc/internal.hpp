@@ -58,11 +58,16 @@ struct LatArgs {
 constexpr int BATCH_MAX = 30;
 template <class A>
 struct Batch {
-    static_assert(sizeof(A) * BATCH_MAX + 4 * (BATCH_MAX + 2) <= 4096, "batch table exceeds the kernel-argument limit");
+    static_assert(sizeof(A) * BATCH_MAX + 4 * (BATCH_MAX + 2) + 8 <= 4096, "batch table exceeds the kernel-argument limit");
     A it[BATCH_MAX];
     int first[BATCH_MAX + 1];
     int n;
+    // if set: the launch does nothing once *halt != 0 (a NaN wall stress earlier in
+    // the same queued sequence of steps)
+    const int *halt = nullptr;
 };
+// true when the launch belongs to a halted step sequence (uniform per launch)
+__device__ __forceinline__ bool halted(const int *halt) { return halt != nullptr && *halt != 0; }
 __device__ __forceinline__ int batch_find(const int *first, int n, int b) {
     int i = 0;
     while (i + 1 < n && b >= first[i + 1]) i++;
@@ -153,6 +158,13 @@ struct dnsg_ctx {
     // f_neq of the 5 layers around each coincident layer [side][19][5][Yf][Xf]
     double *fwall = nullptr, *fneq = nullptr;
 
+    // NaN halt: the wall-stress finaliser sets *d_halt when the stress is NaN, and while
+    // halt_armed (inside dnsg_step / dnsg_run_timed*) every state-changing launch of
+    // the later kernels is a no-op, so the state stays that of the failing step's top
+    int *d_halt = nullptr;
+    bool halt_armed = false;
+    const int *halt() const { return halt_armed ? d_halt : nullptr; }
+
     double force = 0.;  // host mirror of V.force[0]
     long ss = 0;        // next step index of the time loop
     bool have_state = false;
@@ -171,12 +183,14 @@ double *red_upper(dnsg_t *h);
 void rotate_levels(dnsg_t *h);
 
 // launchers (stream-ordered, no host synchronisation)
-hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, int prec);
+hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, int prec, const int *halt = nullptr);
 hipError_t launch_init_lattice(hipStream_t s, dnsg_ctx *h, int grid, void *dst);
+// gl = 1: the host array carries one ghost x-plane and y-row (the reference's in-memory
+// layout); downloads write them as periodic images
 hipError_t launch_aos_to_soa(hipStream_t s, const double *aos, void *soa, int nx, int ny, int nz_aos, int z0,
-                             int nz, int prec);
+                             int nz, int prec, int gl = 0);
 hipError_t launch_soa_to_aos(hipStream_t s, const void *soa, double *aos, int nx, int ny, int nz_aos, int z0,
-                             int nz, int prec);
+                             int nz, int prec, int gl = 0);
 hipError_t launch_sample(hipStream_t s, dnsg_ctx *h, const void *coarse_state, void *lo, void *up);
 hipError_t launch_interface(hipStream_t s, dnsg_ctx *h, int iter);
 // plain injection (ftcdtransfer.c) or, for a handle with p.fltr, the filtered one

@@ -38,6 +38,7 @@ struct SampleItem {
 // item i: workgroups [0, nb) sample the lower side, [nb, 2 nb) the upper side
 template <class S, class P>
 __global__ __launch_bounds__(256) void k_sample(Batch<SampleItem> B) {
+    if (halted(B.halt)) return;
     const int it = batch_find(B.first, B.n, blockIdx.x);
     const SampleItem &A = B.it[it];
     const int n = A.nx * A.ny, nb = (n + 255) / 256;
@@ -93,9 +94,10 @@ static int fill_batch(Batch<A> &B, const A *items, int n, F blocks_of) {
     return blocks;
 }
 
-static hipError_t sample_items(hipStream_t s, int prec, const SampleItem *items, int n) {
+static hipError_t sample_items(hipStream_t s, int prec, const SampleItem *items, int n, const int *halt) {
     for (int i0 = 0; i0 < n; i0 += BATCH_MAX) {
         Batch<SampleItem> B;
+        B.halt = halt;
         const int blocks = fill_batch(B, items + i0, std::min(BATCH_MAX, n - i0),
                                       [](const SampleItem &a) { return 2 * ((a.nx * a.ny + 255) / 256); });
         if (prec == DNSG_PREC_FP64)
@@ -113,7 +115,7 @@ static hipError_t sample_items(hipStream_t s, int prec, const SampleItem *items,
 hipError_t launch_sample(hipStream_t s, dnsg_ctx *h, const void *coarse_state, void *lo, void *up) {
     const Grid &g = h->coarse;
     const SampleItem a = {coarse_state, lo, up, g.nx, g.ny, g.nz - 2};
-    return sample_items(s, h->p.precision, &a, 1);
+    return sample_items(s, h->p.precision, &a, 1, h->halt());
 }
 
 hipError_t launch_sample_many(hipStream_t s, dnsg_ctx *const *hs, int n) {
@@ -122,7 +124,7 @@ hipError_t launch_sample_many(hipStream_t s, dnsg_ctx *const *hs, int n) {
         const Grid &g = hs[i]->coarse;
         v[i] = {g.n(), hs[i]->lev_lo[0], hs[i]->lev_up[0], g.nx, g.ny, g.nz - 2};
     }
-    return n ? sample_items(s, hs[0]->p.precision, v.data(), n) : hipSuccess;
+    return n ? sample_items(s, hs[0]->p.precision, v.data(), n, nullptr) : hipSuccess;
 }
 
 // ---- helpers ---------------------------------------------------------------------------
@@ -315,6 +317,7 @@ struct IfaceItem {
 
 template <class S, class P, int GR, int TX, int TY>
 __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<P> H) {
+    if (halted(BT.halt)) return;
     constexpr int NT = TX * TY, HX = TX + 5, HY = TY + 5, NP = GR * GR;
     constexpr int NH = (HX * HY + NT - 1) / NT;  // halo points per thread
     // The value tile has two sets, used by alternate values: the next value is
@@ -494,7 +497,7 @@ static int iface_groups(long tiles, int slots) {
 }
 
 template <class S, class P, int GR>
-static hipError_t iface_launch(hipStream_t s, IfaceItem *items, int n, HermiteW<P> H) {
+static hipError_t iface_launch(hipStream_t s, IfaceItem *items, int n, HermiteW<P> H, const int *halt) {
     for (int i0 = 0; i0 < n; i0 += BATCH_MAX) {
         const int m = std::min(BATCH_MAX, n - i0);
         long tiles = 0;
@@ -503,6 +506,7 @@ static hipError_t iface_launch(hipStream_t s, IfaceItem *items, int n, HermiteW<
         const int G = iface_groups(tiles, iface_slots<S, P, GR>());
         for (int i = i0; i < i0 + m; i++) items[i].groups = G;
         Batch<IfaceItem> B;
+        B.halt = halt;
         const int blocks = fill_batch(B, items + i0, m, [](const IfaceItem &a) {
             return ((a.Xc + IF_TX - 1) / IF_TX) * ((a.Yc + IF_TY - 1) / IF_TY) * a.groups;
         });
@@ -514,12 +518,12 @@ static hipError_t iface_launch(hipStream_t s, IfaceItem *items, int n, HermiteW<
 }
 
 template <class S, class P>
-static hipError_t iface_items(hipStream_t s, int GR, IfaceItem *items, int n, HermiteW<P> H) {
+static hipError_t iface_items(hipStream_t s, int GR, IfaceItem *items, int n, HermiteW<P> H, const int *halt) {
     switch (GR) {
-    case 1: return iface_launch<S, P, 1>(s, items, n, H);
-    case 2: return iface_launch<S, P, 2>(s, items, n, H);
-    case 3: return iface_launch<S, P, 3>(s, items, n, H);
-    default: return iface_launch<S, P, 4>(s, items, n, H);
+    case 1: return iface_launch<S, P, 1>(s, items, n, H, halt);
+    case 2: return iface_launch<S, P, 2>(s, items, n, H, halt);
+    case 3: return iface_launch<S, P, 3>(s, items, n, H, halt);
+    default: return iface_launch<S, P, 4>(s, items, n, H, halt);
     }
 }
 
@@ -550,10 +554,12 @@ hipError_t launch_interface_many(hipStream_t s, dnsg_ctx *const *hs, int n, int 
     H.exact = pow2_or_zero(H.h00) && pow2_or_zero(H.h10) && pow2_or_zero(H.h01) && pow2_or_zero(H.h11);
     std::vector<IfaceItem> v(2 * (size_t)n);
     for (int i = 0; i < n; i++) iface_pair(hs[i], &v[2 * (size_t)i]);
-    if (prec == DNSG_PREC_FP64) return iface_items<double, double>(s, GR, v.data(), 2 * n, H);
+    // one handle: its own halt flag; a batch of several (the serving queue): none
+    const int *halt = n == 1 ? hs[0]->halt() : nullptr;
+    if (prec == DNSG_PREC_FP64) return iface_items<double, double>(s, GR, v.data(), 2 * n, H, halt);
     HermiteW<float> Hf = {(float)H.h00, (float)H.h10, (float)H.h01, (float)H.h11, H.exact};
-    if (prec == DNSG_PREC_FP32) return iface_items<float, float>(s, GR, v.data(), 2 * n, Hf);
-    return iface_items<__half, float>(s, GR, v.data(), 2 * n, Hf);
+    if (prec == DNSG_PREC_FP32) return iface_items<float, float>(s, GR, v.data(), 2 * n, Hf, halt);
+    return iface_items<__half, float>(s, GR, v.data(), 2 * n, Hf, halt);
 }
 
 hipError_t launch_interface(hipStream_t s, dnsg_ctx *h, int iter) { return launch_interface_many(s, &h, 1, iter); }
@@ -634,6 +640,7 @@ __device__ __forceinline__ void ftcd_node(const FtcdItem &A, int side, int id, d
 // item i: workgroups [0, nb) inject the lower band, [nb, 2 nb) the upper band
 template <class S>
 __global__ __launch_bounds__(256) void k_ftcd(Batch<FtcdItem> B) {
+    if (halted(B.halt)) return;
     const int it = batch_find(B.first, B.n, blockIdx.x);
     const FtcdItem &A = B.it[it];
     const int Xc = A.Xc, Yc = A.Yc, GR = A.GR;
@@ -671,6 +678,7 @@ struct FltrArgs {
     void *coarse;           // coarse post-stream lattice
     double *fneq;           // [side][19][5][Yf][Xf]
     double *psum;           // as FtcdItem::psum
+    const int *halt;        // as Batch::halt
 };
 
 // the 19 values of fine node (i, j, k) of one band in double: f (FP64) or the
@@ -697,6 +705,7 @@ __device__ __forceinline__ void fltr_node(const FltrArgs &A, int side, int i, in
 
 template <class S>
 __global__ __launch_bounds__(256) void k_fltr_slab(FltrArgs A) {
+    if (halted(A.halt)) return;
     const int Xf = A.GR * A.Xc, Yf = A.GR * A.Yc;
     const size_t plane = (size_t)Xf * Yf;
     const size_t id = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -727,6 +736,7 @@ __global__ __launch_bounds__(256) void k_fltr_slab(FltrArgs A) {
 // workgroups [0, nb) filter the lower band, [nb, 2 nb) the upper band (k_ftcd's layout)
 template <class S>
 __global__ __launch_bounds__(256) void k_fltr(FltrArgs A) {
+    if (halted(A.halt)) return;
     const int Xc = A.Xc, Yc = A.Yc, GR = A.GR, Xf = GR * Xc, Yf = GR * Yc;
     const int n = Xc * Yc, nb = (n + 255) / 256;
     const int side = (int)blockIdx.x >= nb ? 1 : 0;
@@ -813,7 +823,8 @@ static hipError_t launch_fltr(hipStream_t s, dnsg_ctx *h, double *psum) {
     if (GR != 4 || zdf - 3 - GR < 0 || !h->fneq || !h->fwall) return hipErrorInvalidValue;
     const size_t nw = (size_t)h->d.Xf * h->d.Yf * Q;
     FltrArgs A = {{h->lower.n(), h->upper.n()}, {h->fwall, h->fwall + nw}, {zdf - 1 - GR, GR}, {0, h->d.zdc - 1},
-                  zdf, h->d.Xc, h->d.Yc, GR, GR * (h->d.omega_f / h->d.omega_c), h->coarse.n(), h->fneq, psum};
+                  zdf, h->d.Xc, h->d.Yc, GR, GR * (h->d.omega_f / h->d.omega_c), h->coarse.n(), h->fneq, psum,
+                  h->halt()};
     const int nb = ftcd_blocks(h);
     if (h->p.precision == DNSG_PREC_FP64) return fltr_launch<double>(s, A, nb);
     if (h->p.precision == DNSG_PREC_FP32) return fltr_launch<float>(s, A, nb);
@@ -839,6 +850,7 @@ hipError_t launch_ftcd_many(hipStream_t s, dnsg_ctx *const *hs, int n, double *p
     const int prec = hs[0]->p.precision;
     for (int i0 = 0; i0 < n; i0 += BATCH_MAX) {
         Batch<FtcdItem> B;
+        B.halt = n == 1 ? hs[0]->halt() : nullptr;
         const int blocks = fill_batch(B, v.data() + i0, std::min(BATCH_MAX, n - i0),
                                       [](const FtcdItem &a) { return 2 * ((a.Xc * a.Yc + 255) / 256); });
         if (prec == DNSG_PREC_FP64)

@@ -261,10 +261,12 @@ struct FluxNextArgs {
     int Xf, Yf, NZs;
     double ub;
     double *force, *enc;
+    const int *halt;                  // as Batch::halt
 };
 
 __global__ __launch_bounds__(RB) void k_flux_seg(FluxNextArgs A, double *seg) {
     __shared__ double lds[4][RB / 64];
+    if (halted(A.halt)) return;
     const int total = A.nc + A.nl + A.nu + A.nf;
     const int e0 = blockIdx.x * ENC_SEG, e1 = min(total, e0 + ENC_SEG);
     double v[4] = {0., 0., 0., 0.};  // momentum, density, coarse momentum, coarse density
@@ -286,6 +288,7 @@ __global__ __launch_bounds__(RB) void k_flux_seg(FluxNextArgs A, double *seg) {
 
 __global__ __launch_bounds__(RB) void k_flux_fin(FluxNextArgs A, const double *seg, int nseg) {
     __shared__ double lds[4][RB / 64];
+    if (halted(A.halt)) return;
     double v[4] = {0., 0., 0., 0.};
     for (int b = threadIdx.x; b < nseg; b += RB)
         for (int k = 0; k < 4; k++) v[k] += seg[b * 4 + k];
@@ -315,6 +318,7 @@ hipError_t launch_flux_next(hipStream_t s, dnsg_ctx *h) {
     A.fs = h->fsum;
     A.Xf = h->d.Xf; A.Yf = h->d.Yf; A.NZs = h->d.NZs; A.ub = h->p.ub;
     A.force = h->d_force; A.enc = h->enc_next;
+    A.halt = h->halt();
     const int nseg = (A.nc + A.nl + A.nu + A.nf + ENC_SEG - 1) / ENC_SEG;
     hipLaunchKernelGGL(k_flux_seg, dim3(nseg), dim3(RB), 0, s, A, h->seg);
     hipLaunchKernelGGL(k_flux_fin, dim3(1), dim3(RB), 0, s, A, (const double *)h->seg, nseg);
@@ -427,6 +431,7 @@ struct WsArgs {
     double *diag;
     double *out;        // [4]: str_up, str_down, stress, pgrad
     const double *enc;  // if set: this step's encalc2 record (G, <rho u>, <rho>, flowrate) for diag[1..4]
+    int *halt;          // if set: becomes 1 when the stress is NaN (wshearstr.c:139-143 aborts there)
 };
 
 __device__ void wshear_finish(const WsArgs &A, const double *v);
@@ -476,6 +481,7 @@ __device__ void wshear_finish(const WsArgs &A, const double *v) {
         const double stress = 0.5 * (stup + stdown);                       // :284
         if (A.diag) { A.diag[5] = stup; A.diag[6] = stdown; A.diag[7] = stress; A.diag[8] = pgrad; }
         if (A.out) { A.out[0] = stup; A.out[1] = stdown; A.out[2] = stress; A.out[3] = pgrad; }
+        if (A.halt && isnan(stress)) A.halt[0] = 1;
     }
 }
 
@@ -486,6 +492,7 @@ hipError_t launch_wshear_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec, con
     WsArgs A;
     A.part = nullptr; A.Xf = h->d.Xf; A.Yf = h->d.Yf; A.NZs = h->d.NZs; A.GR = h->d.GR; A.omega_f = h->d.omega_f;
     A.force = h->d_force; A.diag = diag_rec; A.out = nullptr; A.enc = enc_rec;
+    A.halt = h->halt_armed ? h->d_halt : nullptr;
     hipLaunchKernelGGL(k_fin_wshear_fused, dim3(1), dim3(RB), 0, s, A, pl, pu, h->lower.nz, cpr);
     return hipGetLastError();
 }
@@ -503,7 +510,7 @@ hipError_t launch_wshear(hipStream_t s, dnsg_ctx *h, double *diag_rec) {
         hipLaunchKernelGGL(k_wshear_rows<__half>, grd, blk, 0, s, h->lower.c(), h->upper.c(), l.nx, l.ny, l.nz, part);
     WsArgs A;
     A.part = part; A.Xf = h->d.Xf; A.Yf = h->d.Yf; A.NZs = h->d.NZs; A.GR = h->d.GR; A.omega_f = h->d.omega_f;
-    A.force = h->d_force; A.diag = diag_rec; A.out = nullptr; A.enc = nullptr;
+    A.force = h->d_force; A.diag = diag_rec; A.out = nullptr; A.enc = nullptr; A.halt = nullptr;
     hipLaunchKernelGGL(k_fin_wshear, dim3(1), dim3(RB), 0, s, A);
     return hipGetLastError();
 }

@@ -282,6 +282,7 @@ __device__ __forceinline__ RowSums block_sums(RowSums s) {
 template <class S, bool DS>
 __global__ __launch_bounds__(CS_THREADS) void k_collide_stream(Batch<LatArgs> P) {
     using T = typename ComputeOf<S>::T;
+    if (halted(P.halt)) return;
     const int bg = xcd_remap(blockIdx.x, gridDim.x);
     const int it = batch_find(P.first, P.n, bg);
     const LatArgs &L = P.it[it];
@@ -345,9 +346,10 @@ __global__ __launch_bounds__(CS_THREADS) void k_collide_stream(Batch<LatArgs> P)
     }
 }
 
-hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, int prec) {
+hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, int prec, const int *halt) {
     for (int i0 = 0; i0 < nlat; i0 += BATCH_MAX) {
         Batch<LatArgs> P;
+        P.halt = halt;
         P.n = nlat - i0 < BATCH_MAX ? nlat - i0 : BATCH_MAX;
         int blocks = 0;
         for (int i = 0; i < BATCH_MAX; i++) {
@@ -460,59 +462,65 @@ hipError_t launch_init_lattice(hipStream_t s, dnsg_ctx *h, int grid, void *dst) 
     return hipGetLastError();
 }
 
-// ---- AoS (reference file layout [x][y][k][19]) <-> SoA -----------------------------
+// ---- AoS (reference layouts [x][y][k][19]) <-> SoA ----------------------------------
+// gl = 0: the file layout [nx][ny][nz_aos][19]; gl = 1: the in-memory layout
+// [nx+1][ny+1][nz_aos][19] whose last x-plane and y-row are ghost cells.
 template <class S>
 __global__ __launch_bounds__(256) void k_aos_to_soa(const double *__restrict__ aos, S *__restrict__ soa, int nx,
-                                                    int ny, int nz_aos, int z0, int nz) {
+                                                    int ny, int nz_aos, int z0, int nz, int gl) {
     const size_t n = (size_t)nx * ny * nz * Q;
     const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= n) return;
     const int x = (int)(id % nx), q = (int)((id / nx) % Q);
     const size_t row = id / ((size_t)nx * Q);
     const int y = (int)(row % ny), z = (int)(row / ny);
-    const double v = aos[(((size_t)x * ny + y) * nz_aos + (z + z0)) * Q + q];
+    const double v = aos[(((size_t)x * (ny + gl) + y) * nz_aos + (z + z0)) * Q + q];
     if constexpr (sizeof(S) == 8) soa[id] = v;
     else store_s(soa + id, (float)(v - WQ[q]));
 }
 
+// every node of the host layout, ghost cells as their periodic images
 template <class S>
 __global__ __launch_bounds__(256) void k_soa_to_aos(const S *__restrict__ soa, double *__restrict__ aos, int nx,
-                                                    int ny, int nz_aos, int z0, int nz) {
-    const size_t n = (size_t)nx * ny * nz * Q;
+                                                    int ny, int nz_aos, int z0, int nz, int gl) {
+    const int mx = nx + gl, my = ny + gl;
+    const size_t n = (size_t)mx * my * nz * Q;
     const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= n) return;
-    const int x = (int)(id % nx), q = (int)((id / nx) % Q);
-    const size_t row = id / ((size_t)nx * Q);
-    const int y = (int)(row % ny), z = (int)(row / ny);
+    const int x = (int)(id % mx), q = (int)((id / mx) % Q);
+    const size_t r = id / ((size_t)mx * Q);
+    const int y = (int)(r % my), z = (int)(r / my);
+    const int xs = x < nx ? x : x - nx, ys = y < ny ? y : y - ny;
+    const size_t src = lat_at(q, (size_t)z * ny + ys, xs, nx);
     double v;
-    if constexpr (sizeof(S) == 8) v = soa[id];
-    else v = (double)load_s(soa + id) + WQ[q];
-    aos[(((size_t)x * ny + y) * nz_aos + (z + z0)) * Q + q] = v;
+    if constexpr (sizeof(S) == 8) v = soa[src];
+    else v = (double)load_s(soa + src) + WQ[q];
+    aos[(((size_t)x * my + y) * nz_aos + (z + z0)) * Q + q] = v;
 }
 
 hipError_t launch_aos_to_soa(hipStream_t s, const double *aos, void *soa, int nx, int ny, int nz_aos, int z0, int nz,
-                             int prec) {
+                             int prec, int gl) {
     const size_t n = (size_t)nx * ny * nz * Q;
     dim3 grd((unsigned)((n + 255) / 256)), blk(256);
     if (prec == DNSG_PREC_FP64)
-        hipLaunchKernelGGL(k_aos_to_soa<double>, grd, blk, 0, s, aos, (double *)soa, nx, ny, nz_aos, z0, nz);
+        hipLaunchKernelGGL(k_aos_to_soa<double>, grd, blk, 0, s, aos, (double *)soa, nx, ny, nz_aos, z0, nz, gl);
     else if (prec == DNSG_PREC_FP32)
-        hipLaunchKernelGGL(k_aos_to_soa<float>, grd, blk, 0, s, aos, (float *)soa, nx, ny, nz_aos, z0, nz);
+        hipLaunchKernelGGL(k_aos_to_soa<float>, grd, blk, 0, s, aos, (float *)soa, nx, ny, nz_aos, z0, nz, gl);
     else
-        hipLaunchKernelGGL(k_aos_to_soa<__half>, grd, blk, 0, s, aos, (__half *)soa, nx, ny, nz_aos, z0, nz);
+        hipLaunchKernelGGL(k_aos_to_soa<__half>, grd, blk, 0, s, aos, (__half *)soa, nx, ny, nz_aos, z0, nz, gl);
     return hipGetLastError();
 }
 
 hipError_t launch_soa_to_aos(hipStream_t s, const void *soa, double *aos, int nx, int ny, int nz_aos, int z0, int nz,
-                             int prec) {
-    const size_t n = (size_t)nx * ny * nz * Q;
+                             int prec, int gl) {
+    const size_t n = (size_t)(nx + gl) * (ny + gl) * nz * Q;
     dim3 grd((unsigned)((n + 255) / 256)), blk(256);
     if (prec == DNSG_PREC_FP64)
-        hipLaunchKernelGGL(k_soa_to_aos<double>, grd, blk, 0, s, (const double *)soa, aos, nx, ny, nz_aos, z0, nz);
+        hipLaunchKernelGGL(k_soa_to_aos<double>, grd, blk, 0, s, (const double *)soa, aos, nx, ny, nz_aos, z0, nz, gl);
     else if (prec == DNSG_PREC_FP32)
-        hipLaunchKernelGGL(k_soa_to_aos<float>, grd, blk, 0, s, (const float *)soa, aos, nx, ny, nz_aos, z0, nz);
+        hipLaunchKernelGGL(k_soa_to_aos<float>, grd, blk, 0, s, (const float *)soa, aos, nx, ny, nz_aos, z0, nz, gl);
     else
-        hipLaunchKernelGGL(k_soa_to_aos<__half>, grd, blk, 0, s, (const __half *)soa, aos, nx, ny, nz_aos, z0, nz);
+        hipLaunchKernelGGL(k_soa_to_aos<__half>, grd, blk, 0, s, (const __half *)soa, aos, nx, ny, nz_aos, z0, nz, gl);
     return hipGetLastError();
 }
 

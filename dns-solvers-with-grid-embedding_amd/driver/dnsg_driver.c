@@ -492,8 +492,15 @@ int main(int argc, char **argv) {
         next = ((ss / ut) + 1) * ut;
         if (next > up + 1) next = up + 1;
         n = next - ss;
-        dg = malloc(sizeof(dnsg_diag) * (size_t)n);
+        dg = calloc((size_t)n, sizeof(dnsg_diag));
+        if (!dg) { fprintf(stderr, "out of memory\n"); return 1; }
         rc = dnsg_step(h, (int)n, dg);
+        if (rc && rc != DNSG_ERR_NAN) { /* no record of an unfinished batch is logged */
+            free(dg);
+            die(h, "dnsg_step", rc);
+        }
+        /* on NaN the records end at the failing step, whose flowrate/density lines
+           encalc2 has written before wshearstr aborts (wshearstr.c:139-143) */
         for (s = 0; s < n; s++) {
             if (rc == DNSG_ERR_NAN && isnan(dg[s].stress)) {
                 append_log("flowrate.txt", dg[s].time, dg[s].flowrate);

@@ -159,6 +159,15 @@ int dnsg_destroy(dnsg_t *h);
 const char *dnsg_last_error(const dnsg_t *h);
 int dnsg_get_dims(const dnsg_t *h, dnsg_dims *d);
 
+/* The node-code arrays V.s, V.sls and V.sus as solid_init (solid_init.c:21-133) and
+ * the pattern routine (slip_ridges_init.c:15-81, slip_posts_init.c:15-132,
+ * slip_spanwise_init.c:15-80) leave them on one rank (main:964-973), read back
+ * from what this handle's kernels use: s_c [Xc+1][Yc+1][zdc], s_l and s_u
+ * [Xf+1][Yf+1][zdf] (the ghosted extents of main:145-151), codes -1 no-slip wall,
+ * -2 slip wall, 0 fluid, 1 periodic or bounded (solid_init.c:14-19).  Any pointer
+ * may be NULL. */
+int dnsg_node_codes(dnsg_t *h, int *s_c, int *s_l, int *s_u);
+
 /* ---- state ---------------------------------------------------------------- */
 
 /* '-s' mode init (init.c:15-56) or the seeded laminar+noise state, including the
@@ -171,6 +180,23 @@ int dnsg_init_state(dnsg_t *h);
 int dnsg_upload_state(dnsg_t *h, const double *coarse, const double *lower,
                       const double *upper, const double *lin3, const double *uin3,
                       double force);
+
+/* The same state in the reference's in-memory arrays, so that the reference's main
+ * can hand its own POINTER members over (V is passed by value, definitions.h:161-229):
+ *   ftemp               V.ftemp    [Xc+1][Yc+1][zdc][19]  (main:145-146, 489-494)
+ *   slftemp, suftemp    V.slftemp, V.suftemp [Xf+1][Yf+1][zdf][19]  (main:149-151)
+ *   lin[3]              {V.lin, V.linp, V.linpp}, uin[3] {V.uin, V.uinp, V.uinpp},
+ *                       each [Xc+6][Yc+6][23], interior at (i+2, j+2) (Fbi,
+ *                       definitions.h:132; main:827-866)
+ * The last x-plane and y-row of a lattice and the 2-cell ring (4 cells on the high
+ * side) of a level are ghost cells: the upload reads only the interior, the download
+ * writes every ghost cell as its periodic image, which is what the reference's halo
+ * exchange puts there on one rank.  Any pointer may be NULL, as above. */
+int dnsg_upload_state_ghosted(dnsg_t *h, const double *ftemp, const double *slftemp,
+                              const double *suftemp, const double *const *lin,
+                              const double *const *uin, double force);
+int dnsg_download_state_ghosted(dnsg_t *h, double *ftemp, double *slftemp, double *suftemp,
+                                double *const *lin, double *const *uin, double *force);
 
 /* '-i' mode (main:982-997): replaces initializer() (initializer.c:15-545, the ftemp
  * pass, with sp_interpolate_qd.c) and inter_init() (inter_init.c:13-271).  `uniform`

@@ -313,6 +313,60 @@ def test_nan_in_state_aborts_with_status():
     assert ei.value.rc == -3
 
 
+@pytest.mark.parametrize("timed", [False, True])
+def test_nan_stops_at_the_failing_step(timed):
+    """A NaN planted in the middle of the coarse grid reaches the near-wall band layers
+    a few steps later.  The oracle stops at the top of the step whose wall stress is
+    NaN, as the reference aborts there (wshearstr.c:139-143).  The GPU run, asked for
+    many more steps in one call, returns the NaN status with the records up to that
+    step equal to the oracle's, and holds the state of that step's top: every lattice
+    and interface level as the oracle's (constant force, bitwise, NaNs in the same
+    places), and a further call fails again at the same step."""
+    import ctypes
+    kw = dict(CASES["rslip"], force_mode="const", g_const=2e-5)
+    o, s = pair(kw)
+    st = o.state()
+    st["coarse"][8, 4, 6, 0] = np.nan
+    s.upload(st["coarse"], st["lower"], st["upper"], st["lin"], st["uin"], st["force"])
+    o2 = Oracle(dnsg.make_params(**kw, **INIT), threads=4)
+    o2.init_seeded()
+    for a, aid in (("coarse", 1), ("lower", 3), ("upper", 5)):
+        o2.array(aid)[...] = st[a]
+    o2.set_levels(0, st["lin"])
+    o2.set_levels(1, st["uin"])
+    recs, fail = [], None
+    for ss in range(40):
+        rc, d = o2.step(ss)
+        recs.append(d)
+        if rc:
+            assert rc == -3
+            fail = ss
+            break
+    assert fail is not None and fail >= 2, fail
+    n = 60
+    d = (dnsg.Diag * n)()
+    if timed:
+        ms = ctypes.c_double()
+        rc = s.lib.dnsg_run_timed(s.h, n, d, ctypes.byref(ms))
+    else:
+        rc = s.lib.dnsg_step(s.h, n, d)
+    print(f"NaN planted: oracle fails at step {fail}, GPU rc {rc}", flush=True)
+    assert rc == -3
+    assert f"step {fail}" in s.lib.dnsg_last_error(s.h).decode()
+    for k in range(fail + 1):
+        for f in ("time", "force", "density", "flowrate"):
+            a, b = getattr(d[k], f), getattr(recs[k], f)
+            assert (np.isnan(a) and np.isnan(b)) or a == b, (k, f, a, b)
+    assert np.isnan(d[fail].stress) and not np.isnan(d[fail - 1].stress)
+    g, ref = s.download(), o2.state()
+    for k in GRIDS + ("lin", "uin"):
+        assert np.array_equal(g[k], ref[k], equal_nan=True), k
+    assert np.isnan(g["coarse"]).any()
+    d1 = (dnsg.Diag * 1)()
+    assert s.lib.dnsg_step(s.h, 1, d1) == -3
+    assert d1[0].time == d[fail].time
+
+
 def test_running_statistics_match_host_average():
     """time averages = host average of the per-snapshot profiles (law of total variance)"""
     o, s = pair(CASES["rslip"])
