@@ -29,10 +29,20 @@ import os
 import sys
 import time
 
-REPO = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, os.path.join(REPO, "tests"))
+import numpy as np
 
-import numpy as np  # noqa: E402
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(REPO, "dns-solvers-with-grid-embedding_amd")
+
+
+def load_pkg():
+    """the product binding (ctypes over libdnsg.so), loaded by path: the package
+    directory's name is not a Python identifier"""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("dnsg_amd", os.path.join(PKG_DIR, "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
 
 METRIC = "GLUPS + %HBM roofline, FP32 vs FP64 at Re_tau=180; p50/p99 slip-queue latency"
 HBM_PEAK_GBS = 8000.0
@@ -53,6 +63,69 @@ def workload_params(mod, prec="fp64", force_mode="const"):
                            pattern="rslip", gyplussize=32, wyplussize=32, force_mode=force_mode,
                            g_const=G, precision=prec, init_kind="laminar_noise", seed=2024,
                            u_max=1.5 * ub, noise=0.10, CFL=0.1)
+
+
+def cfg5_params(mod, prec="fp32", xdv=1024, ydv=512):
+    """The Re_tau = 395 channel with square superhydrophobic posts: coarse 1024 (streamwise)
+    x 512 (spanwise) x 256 (wall-normal, zdv incl. the two overlap layers), factor-2 bands
+    of 32 fluid layers (fine 2048 x 1024 x 33 each), solid posts of 32 x 32 fine (16 x 16
+    coarse) cells in a 64 x 64 fine period on both walls (solid fraction 0.25), shifted
+    FP32 storage, the Re_tau = 180 channel's tau / forcing rule (u_tau = 0.005,
+    nu = u_tau h / 395, tau_c = 3 nu + 1/2, G = u_tau^2 / h) and its seeded +-10% noise on
+    the laminar profile at Dean's bulk velocity (U_b = 17.4 u_tau at Re_tau = 395).
+    402,653,184 LU per coarse step; about 41 GB of FP32 lattices.  xdv, ydv shrink the
+    periodic extents only (multiples of 32 keep the post pattern whole)."""
+    GR, zds, zdv = 2, 32, 256
+    zdf = zds + 1
+    NZs = 2 * zdf + GR * (zdv - 4) + GR - 1
+    h = (NZs - 2.0) / (2.0 * GR)
+    u_tau = 0.005
+    nu = u_tau * h / 395.0
+    tau_c = 3.0 * nu + 0.5
+    G = u_tau * u_tau / h
+    ub = 17.4 * u_tau
+    return mod.make_params(xdv=xdv, ydv=ydv, zdv=zdv, gratio=GR, zds=zds, tau_c=tau_c, ub=ub,
+                           pattern="pslip", gxplussize=32, wxplussize=32, gyplussize=32, wyplussize=32,
+                           force_mode="const", g_const=G, precision=prec, init_kind="laminar_noise",
+                           seed=395, u_max=1.5 * ub, noise=0.10, CFL=0.1)
+
+
+def run_cfg5(mod, steps, warmup):
+    """cfg-5 leg: GLUPS of the full Re_tau = 395 posts channel in shifted FP32, the roofline
+    of its band launch (HIP events on the solver's stream inside the timed steps), and the
+    cost of the three grids' 1-D spectra, which the run takes every 500 steps."""
+    p = cfg5_params(mod)
+    s = mod.Solver(p)
+    s.init_state()
+    if warmup:
+        s.run_timed(warmup)
+    t0 = time.perf_counter()
+    ms_ev, diag, sec = s.run_timed(steps, sections="band")
+    wall = time.perf_counter() - t0
+    d = s.dims
+    lu = lu_per_step(d)
+    band_lu = s.band_launch_substeps() * 2 * d.Xf * d.Yf * (d.zdf - 1)
+    band_bytes = band_lu * BYTES_PER_LU["fp32"]
+    gbs = band_bytes / (sec["band"] * 1e-3) / 1e9 if sec["band"] > 0 else None
+    t1 = time.perf_counter()
+    for g in range(3):
+        ex, ey = s.spectra(g)
+        if not (np.isfinite(ex).all() and np.isfinite(ey).all()):
+            raise RuntimeError("cfg 5: non-finite spectra")
+    spec_s = time.perf_counter() - t1
+    ms_step = wall * 1e3 / steps
+    out = dict(workload="Re_tau=395 square-post channel, coarse 1024x512x256, factor-2 bands 2x2048x1024x33, "
+                        "posts 32x32 fine in a 64x64 period (solid fraction 0.25), shifted FP32, constant force",
+               lu_per_step=lu, glups=round(lu * steps / wall / 1e9, 4), ms_per_step=round(ms_step, 4),
+               steps=steps, warmup=warmup, hbm_roofline_frac_step=round(lu * steps / wall / 1e9 * 152 / HBM_PEAK_GBS, 4),
+               band_launch=dict(lu=band_lu, bytes=band_bytes, ms=round(sec["band"], 5),
+                                gbs=round(gbs, 1) if gbs else None,
+                                frac=round(gbs / HBM_PEAK_GBS, 4) if gbs else None),
+               spectra_s_three_grids=round(spec_s, 4),
+               spectra_share_at_every_500_steps=round(spec_s / (500 * ms_step * 1e-3), 5),
+               flowrate_last=float(diag["flowrate"][-1]), mass=s.mass())
+    s.close()
+    return out
 
 
 def lu_per_step(d):
@@ -76,10 +149,12 @@ def run_gpu(mod, prec, steps, warmup, force_mode="const"):
     out = dict(glups=lu * steps / wall / 1e9, ms_per_step=wall * 1e3 / steps, ms_per_step_events=ms_ev / steps,
                kernels_ms=kt, lu_per_step=lu, flowrate_last=float(diag["flowrate"][-1]),
                stress_last=float(diag["stress"][-1]))
-    band_lu = 2 * d.Xf * d.Yf * (d.zdf - 1)       # both bands, one launch
+    subs = s.band_launch_substeps()                # 2: both gratio-2 substeps in one launch
+    band_lu = subs * 2 * d.Xf * d.Yf * (d.zdf - 1)  # both bands, every substep the launch covers
     band_bytes = band_lu * BYTES_PER_LU[prec]
     band_ms = sec["band"]
-    out["band_launch"] = dict(lu=band_lu, bytes=band_bytes, ms=band_ms, launches=steps * (d.GR - 1),
+    out["band_launch"] = dict(lu=band_lu, bytes=band_bytes, ms=band_ms, substeps=subs,
+                              launches=steps * (1 if subs == 2 else d.GR - 1),
                               gbs=band_bytes / (band_ms * 1e-3) / 1e9 if band_ms > 0 else None)
     s.close()
     return out
@@ -102,9 +177,16 @@ def traffic_from_profiles():
 
 
 def cpu_baseline(mod, nsteps):
-    from helpers import Oracle  # the oracle: C restatement of the reference step
-    cores = min(16, int(os.environ.get("OMP_NUM_THREADS", "16")), os.cpu_count() or 16)
-    p = workload_params(mod, "fp64")
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from helpers import Oracle  # the oracle: C restatement of the reference step (the checker)
+    from helpers import dnsg as omod  # its parameter structs come from the checker's own binding
+    # threads: every core this process may run on, capped by the host's CPU share for one
+    # GPU (OMP_NUM_THREADS where the host sets it); nproc and the affinity are reported
+    nproc = os.cpu_count() or 1
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    share = int(os.environ.get("OMP_NUM_THREADS", str(avail)) or avail)
+    cores = max(1, min(avail, share))
+    p = workload_params(omod, "fp64")
     o = Oracle(p, threads=cores)
     o.init_seeded()
     o.step(0)  # first touch of every page outside the timed sample
@@ -114,9 +196,10 @@ def cpu_baseline(mod, nsteps):
     dt = time.perf_counter() - t0
     lu = lu_per_step(o.dims)
     o.close()
-    return dict(value=lu * nsteps / dt / 1e9, unit="GLUPS", cores=cores, kind="port",
+    return dict(value=lu * nsteps / dt / 1e9, unit="GLUPS", cores=cores, kind="port", nproc=nproc,
+                affinity_cores=avail, omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
                 sample=f"{nsteps} coarse steps of the same 512x256x128 + 2x1024x512x17 channel "
-                       f"(oracle, C restatement of the reference step, OpenMP, {dt:.1f} s)")
+                       f"(oracle, C restatement of the reference step, OpenMP on {cores} threads, {dt:.1f} s)")
 
 
 def cfg_small_params(mod, prec):
@@ -254,6 +337,8 @@ def main():
     ap.add_argument("--queue-check", type=int, default=8, help="requests re-solved in FP64 for the slip check")
     ap.add_argument("--no-queue", action="store_true")
     ap.add_argument("--no-precision-report", action="store_true")
+    ap.add_argument("--no-cfg5", action="store_true")
+    ap.add_argument("--cfg5-steps", type=int, default=10)
     ap.add_argument("--pmc-run", action="store_true",
                     help="minimal run for hardware-counter passes: 3 FP64 steps of the workload, then one "
                          "1 GiB copy-probe launch whose known bytes calibrate FETCH_SIZE / WRITE_SIZE")
@@ -262,7 +347,7 @@ def main():
         print("bench.py: this build runs on one GPU only (the lattices of one channel live on a single "
               "MI355X); multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
-    from helpers import dnsg as mod
+    mod = load_pkg()
 
     if args.pmc_run:
         s = mod.Solver(workload_params(mod, "fp64"))
@@ -294,8 +379,16 @@ def main():
     traffic = traffic_from_profiles()
     roof = dict(bound="hbm", achieved=bl["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=(bl["gbs"] / HBM_PEAK_GBS) if bl["gbs"] else None, traffic=traffic,
-                kernel="k_collide_stream<double, false> (both wall bands, one launch per substep)",
+                kernel=("k_band2<double, false> (both wall bands, both gratio-2 substeps in one launch)"
+                        if bl.get("substeps") == 2 else
+                        "k_collide_stream<double, false> (both wall bands, one launch per substep)"),
                 algorithmic_bytes_per_launch=bl["bytes"], launch_ms=bl["ms"])
+    cfg5 = None
+    if not args.no_cfg5:
+        print(f"bench: Re_tau=395 posts channel, fp32, {args.cfg5_steps} steps", file=sys.stderr, flush=True)
+        cfg5 = run_cfg5(mod, args.cfg5_steps, 2)
+        print(f"bench: Re_tau=395 {cfg5['glups']} GLUPS, band {cfg5['band_launch']['gbs']} GB/s, "
+              f"spectra {cfg5['spectra_s_three_grids']} s", file=sys.stderr, flush=True)
     prec_report = None
     if not args.no_precision_report:
         print("bench: precision tier report (64x32x32 channel, 500 steps)", file=sys.stderr, flush=True)
@@ -332,6 +425,7 @@ def main():
         "fp64_flux_controller": flux,
         "slip_queue": queue,
         "precision_tier": prec_report,
+        "retau395_posts_fp32": cfg5,
         "hbm_probe_gbs": probe,
         "tiers": {k: {"glups": round(v["glups"], 4), "ms_per_step": round(v["ms_per_step"], 4),
                       "hbm_frac": round(v["glups"] * BYTES_PER_LU[k] / HBM_PEAK_GBS, 4),

@@ -94,6 +94,9 @@ SIGNATURES = {
     "dnsg_upload_state": [_P, _DP, _DP, _DP, _DP, _DP, ctypes.c_double],
     "dnsg_init_uniform": [_P, _DP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "dnsg_budget2d": [_P, _DP, _DP, _DP, _DP, _DP],
+    "dnsg_budget2d_accumulate": [_P, _DP, _DP, _DP],
+    "dnsg_budget2d_average": [_P, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_long)],
+    "dnsg_budget2d_reset": [_P],
     "dnsg_download_state": [_P, _DP, _DP, _DP, _DP, _DP, _DP],
     "dnsg_upload_state_ghosted": [_P, _DP, _DP, _DP, ctypes.POINTER(_DP), ctypes.POINTER(_DP), ctypes.c_double],
     "dnsg_download_state_ghosted": [_P, _DP, _DP, _DP, ctypes.POINTER(_DP), ctypes.POINTER(_DP), _DP],
@@ -101,6 +104,7 @@ SIGNATURES = {
     "dnsg_run_timed": [_P, ctypes.c_int, ctypes.POINTER(Diag), _DP],
     "dnsg_run_timed_sections": [_P, ctypes.c_int, ctypes.POINTER(Diag), _DP, _DP, ctypes.c_int],
     "dnsg_kernel_times": [_P, ctypes.c_int, _DP],
+    "dnsg_band_launch_substeps": [_P, ctypes.POINTER(ctypes.c_int)],
     "dnsg_hbm_probe": [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, _DP],
     "dnsg_encalc2": [_P, ctypes.c_long, ctypes.POINTER(Diag)],
     "dnsg_wshearstr": [_P, ctypes.c_long, ctypes.POINTER(Diag)],
@@ -122,6 +126,7 @@ SIGNATURES = {
     "dnsg_queue_destroy": [_P],
     "dnsg_queue_submit": [_P, ctypes.POINTER(Params), ctypes.c_uint64, ctypes.c_double],
     "dnsg_queue_advance": [_P, ctypes.c_int, ctypes.POINTER(SlipResult), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+    "dnsg_queue_graph_launches": [_P, ctypes.POINTER(ctypes.c_long)],
     "dnsg_queue_info": [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                         ctypes.POINTER(ctypes.c_size_t)],
 }
@@ -278,6 +283,25 @@ class Solver:
         self._check(self.lib.dnsg_step(self.h, n, d))
         return diag_array(d, n)
 
+    def budget2d_accumulate(self):
+        """one snapshot's budget profiles (turb [19][NZ], vort [3][NZ], ke [30][NZ]),
+        also folded into the handle's time averages"""
+        NZ = self.dims.NZ
+        turb, vort, ke = np.empty((19, NZ)), np.empty((3, NZ)), np.empty((30, NZ))
+        self._check(self.lib.dnsg_budget2d_accumulate(self.h, _ptr(turb), _ptr(vort), _ptr(ke)))
+        return turb, vort, ke
+
+    def budget2d_average(self):
+        """time averages of the folded snapshots (rms rows as mean squares) and their count"""
+        NZ = self.dims.NZ
+        turb, vort, ke = np.empty((19, NZ)), np.empty((3, NZ)), np.empty((30, NZ))
+        cnt = ctypes.c_long()
+        self._check(self.lib.dnsg_budget2d_average(self.h, _ptr(turb), _ptr(vort), _ptr(ke), ctypes.byref(cnt)))
+        return turb, vort, ke, cnt.value
+
+    def budget2d_reset(self):
+        self._check(self.lib.dnsg_budget2d_reset(self.h))
+
     def run_timed(self, n, sections=None):
         """n steps without per-step host synchronisation: (device ms, diag).  With
         sections="all" (or one name of SECTIONS) also the average ms per launch of the
@@ -298,6 +322,12 @@ class Solver:
         ms = np.zeros(5)
         self._check(self.lib.dnsg_kernel_times(self.h, n, _ptr(ms)))
         return dict(zip(SECTIONS, ms))
+
+    def band_launch_substeps(self):
+        """band substeps one band launch covers (2: both gratio-2 substeps in one pass)"""
+        n = ctypes.c_int()
+        self._check(self.lib.dnsg_band_launch_substeps(self.h, ctypes.byref(n)))
+        return n.value
 
     def mass(self):
         m = ctypes.c_double()
@@ -413,6 +443,12 @@ class Queue:
         nd = ctypes.c_int()
         self._check(self.lib.dnsg_queue_advance(self.q, steps, buf, cap, ctypes.byref(nd)))
         return [buf[i].asdict() for i in range(nd.value)]
+
+    def graph_launches(self):
+        """check intervals run as a replayed HIP graph so far"""
+        n = ctypes.c_long()
+        self._check(self.lib.dnsg_queue_graph_launches(self.q, ctypes.byref(n)))
+        return n.value
 
     def info(self):
         p, a, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()

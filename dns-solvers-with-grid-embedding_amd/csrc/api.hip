@@ -187,13 +187,14 @@ void band_args(dnsg_t *h, LatArgs *L) {
 
 double *red_coarse(dnsg_t *h) { return h->red; }
 double *red_lower(dnsg_t *h) { return h->red + (size_t)h->coarse.ny * h->coarse.nz * cs_chunks(h->coarse.nx) * RED_W; }
-double *red_upper(dnsg_t *h) { return red_lower(h) + (size_t)h->lower.ny * h->lower.nz * cs_chunks(h->lower.nx) * RED_W; }
+double *red_upper(dnsg_t *h) { return red_lower(h) + (size_t)h->lower.ny * h->lower.nz * band_cpr(h) * RED_W; }
 
 void rotate_levels(dnsg_t *h) {  // xcomputations.c:548-556
-    void *t = h->lev_lo[2];
-    h->lev_lo[2] = h->lev_lo[1]; h->lev_lo[1] = h->lev_lo[0]; h->lev_lo[0] = t;
-    t = h->lev_up[2];
-    h->lev_up[2] = h->lev_up[1]; h->lev_up[1] = h->lev_up[0]; h->lev_up[0] = t;
+    // the spare (slot 3) becomes the newest level, the oldest the spare
+    void *t = h->lev_lo[3];
+    h->lev_lo[3] = h->lev_lo[2]; h->lev_lo[2] = h->lev_lo[1]; h->lev_lo[1] = h->lev_lo[0]; h->lev_lo[0] = t;
+    t = h->lev_up[3];
+    h->lev_up[3] = h->lev_up[2]; h->lev_up[2] = h->lev_up[1]; h->lev_up[1] = h->lev_up[0]; h->lev_up[0] = t;
 }
 
 }  // namespace dnsg
@@ -278,13 +279,76 @@ static int sec_average(dnsg_t *h, const SecTimer &T, double *ms) {
 // The coarse collide-stream and the first band substep touch disjoint lattices
 // and read the same force, so they run as one launch over the three lattices
 // (xcomputations and the first xfcomputations of the reference's loop).
+// One coarse step with the two-substep band kernel (gratio 2, k_band2.hip): the
+// coarse collide-stream alone, the sampling and the first fill (into the bands'
+// output lattices), both band substeps in one pass, then the reductions (the bands'
+// sums come from that pass), the second fill and the injection.  Every launch before
+// the wall-stress check writes only the lattices' other buffers and the spare
+// interface level, so a NaN there leaves the state of the step's top intact.
+int do_step_band2(dnsg_t *h, double *rec, SecTimer *T) {
+    const bool flux = h->p.force_mode == DNSG_FORCE_FLUX;
+    const bool from_writes = flux && h->flux_fused;
+    h->band_cols = B2_TX;
+    if (T) T->start_step(h->stream);
+    if (flux && !from_writes) {
+        if (T) T->begin(h->stream, SEC_REDUCE);
+        HIPCHK(h, launch_encalc2(h->stream, h, rec));
+        if (T) T->end(h->stream, SEC_REDUCE);
+    }
+    LatArgs C = coarse_args(h);
+    if (!flux) C.red = red_coarse(h);
+    if (flux) C.dsum = h->dsum;
+    if (T) T->begin(h->stream, SEC_COARSE);
+    HIPCHK(h, launch_collide_stream(h->stream, &C, 1, h->p.precision, h->halt()));
+    if (T) T->end(h->stream, SEC_COARSE);
+    rotate_levels(h);
+    if (T) T->begin(h->stream, SEC_SAMPLE);
+    HIPCHK(h, launch_sample(h->stream, h, h->coarse.n(), h->lev_lo[0], h->lev_up[0]));
+    if (T) T->end(h->stream, SEC_SAMPLE);
+    if (T) T->begin(h->stream, SEC_IFACE);
+    HIPCHK(h, launch_interface(h->stream, h, 1));
+    if (T) T->end(h->stream, SEC_IFACE);
+    LatArgs B[2];
+    band_args(h, B);
+    B[0].red = red_lower(h);
+    B[1].red = red_upper(h);
+    if (flux) {
+        B[0].dsum = h->dsum + 2 * (size_t)h->d.Yc * h->d.zdc * cs_chunks(h->d.Xc);
+        B[1].dsum = B[0].dsum + 2 * (size_t)h->d.Yf * (h->d.zdf - 1) * band_cpr(h);
+    }
+    if (T) T->begin(h->stream, SEC_BAND);
+    HIPCHK(h, launch_band2(h->stream, B, h->p.precision, h->halt()));
+    if (T) T->end(h->stream, SEC_BAND);
+    if (T) T->begin(h->stream, SEC_REDUCE);
+    if (!flux) HIPCHK(h, launch_encalc2_fused(h->stream, h, rec));
+    HIPCHK(h, launch_wshear_fused(h->stream, h, rec, from_writes ? h->enc_next : nullptr));
+    if (T) T->end(h->stream, SEC_REDUCE);
+    if (T) T->begin(h->stream, SEC_IFACE);
+    HIPCHK(h, launch_interface(h->stream, h, 2));
+    HIPCHK(h, launch_ftcd(h->stream, h, flux ? h->fsum : nullptr));
+    if (T) T->end(h->stream, SEC_IFACE);
+    if (flux) {
+        if (T) T->begin(h->stream, SEC_REDUCE);
+        HIPCHK(h, launch_flux_next(h->stream, h));
+        if (T) T->end(h->stream, SEC_REDUCE);
+    }
+    h->lower.swap();
+    h->upper.swap();
+    h->flux_fused = flux;
+    h->coarse.swap();
+    h->ss++;
+    return DNSG_OK;
+}
+
 int do_step(dnsg_t *h, double *rec, SecTimer *T = nullptr) {
+    if (h->band2) return do_step_band2(h, rec, T);
     const bool flux = h->p.force_mode == DNSG_FORCE_FLUX;
     // flux mode: G for this step comes from the previous step's writes when that
     // step ran here (flux_fused), else from a pass over the lattices (first step,
     // or after the state was changed from outside)
     const bool from_writes = flux && h->flux_fused;
     const int GR = h->d.GR;
+    h->band_cols = CS_THREADS;
     if (T) T->start_step(h->stream);
     if (flux && !from_writes) {
         if (T) T->begin(h->stream, SEC_REDUCE);
@@ -292,7 +356,7 @@ int do_step(dnsg_t *h, double *rec, SecTimer *T = nullptr) {
         if (T) T->end(h->stream, SEC_REDUCE);
     }
     double *dl = h->dsum + 2 * (size_t)h->d.Yc * h->d.zdc * cs_chunks(h->d.Xc);
-    double *du = dl + 2 * (size_t)h->d.Yf * (h->d.zdf - 1) * cs_chunks(h->d.Xf);
+    double *du = dl + 2 * (size_t)h->d.Yf * (h->d.zdf - 1) * band_cpr(h);
     for (int iter = 1; iter <= GR; iter++) {
         LatArgs L[3];
         LatArgs *B = L;  // the band pair of this substep
@@ -362,23 +426,55 @@ void fill_diag(const dnsg_t *h, const double *rec, long ss, dnsg_diag *d) {
 // the interface levels, and whether the flux controller's G for it is formed.
 struct StepTop {
     int cur[3];
-    void *lo[3], *up[3];
+    void *lo[4], *up[4];
     bool flux_fused;
 };
 
 StepTop step_top(const dnsg_t *h) {
     StepTop t;
     t.cur[0] = h->coarse.cur; t.cur[1] = h->lower.cur; t.cur[2] = h->upper.cur;
-    for (int l = 0; l < 3; l++) { t.lo[l] = h->lev_lo[l]; t.up[l] = h->lev_up[l]; }
+    for (int l = 0; l < 4; l++) { t.lo[l] = h->lev_lo[l]; t.up[l] = h->lev_up[l]; }
     t.flux_fused = h->flux_fused;
+    return t;
+}
+
+// the top of the step s steps after top t: do_step swaps the coarse buffers once and
+// the band buffers GR times (once with the two-substep band kernel), rotates the
+// interface levels once, and leaves the flux controller's G formed from its writes
+// in flux mode
+StepTop top_after(const dnsg_t *h, StepTop t, long s) {
+    const int band_swaps = h->band2 ? 1 : h->d.GR;
+    for (long k = 0; k < s; k++) {
+        t.cur[0] ^= 1;
+        if (band_swaps & 1) { t.cur[1] ^= 1; t.cur[2] ^= 1; }
+        void *a = t.lo[3]; t.lo[3] = t.lo[2]; t.lo[2] = t.lo[1]; t.lo[1] = t.lo[0]; t.lo[0] = a;
+        a = t.up[3]; t.up[3] = t.up[2]; t.up[2] = t.up[1]; t.up[1] = t.up[0]; t.up[0] = a;
+        t.flux_fused = h->p.force_mode == DNSG_FORCE_FLUX;
+    }
     return t;
 }
 
 void restore_top(dnsg_t *h, const StepTop &t) {
     h->coarse.cur = t.cur[0]; h->lower.cur = t.cur[1]; h->upper.cur = t.cur[2];
-    for (int l = 0; l < 3; l++) { h->lev_lo[l] = t.lo[l]; h->lev_up[l] = t.up[l]; }
+    for (int l = 0; l < 4; l++) { h->lev_lo[l] = t.lo[l]; h->lev_up[l] = t.up[l]; }
     h->flux_fused = t.flux_fused;
 }
+
+// clears the halt flag with a kernel rather than a memset node, so a caller that
+// captures the queued steps into a graph replays the clear too
+__global__ void k_clear_flag(int *f) {
+    if (threadIdx.x == 0) f[0] = 0;
+}
+
+// Where the queued steps keep their records and halt flag.  dnsg_step and the timed
+// runs use the handle's own buffers (grown on demand) and pinned staging;
+// dnsg_step_noalloc passes the device's static scratch array and copies the records
+// back through a fixed host buffer on the stack.
+struct StepBufs {
+    double *rec;  // device, DIAG_WORDS per queued step
+    int *halt;    // device halt flag
+    bool pinned;  // records come back through h->h_pinned (else in chunks through the stack)
+};
 
 // Queue m steps without host synchronisation, then copy the m diag records back.
 // A NaN wall stress in step s (wshearstr.c:139-143 aborts there) sets the device
@@ -386,11 +482,18 @@ void restore_top(dnsg_t *h, const StepTop &t) {
 // at once; the host then rewinds its buffer and level pointers to the top of step
 // s, so the state, ss and force are those the reference holds when it aborts, and
 // the records up to and including step s are filled.  With ms, the HIP-event time
-// of the m steps; with T, section events inside them.
-int run_steps(dnsg_t *h, int m, dnsg_diag *diag, float *ms, SecTimer *T) {
+// of the m steps; with T, section events inside them.  bufs = null: the handle's
+// own buffers.
+int run_steps(dnsg_t *h, int m, dnsg_diag *diag, float *ms, SecTimer *T, const StepBufs *bufs = nullptr) {
     int rc;
-    if ((rc = ensure_diag(h, m))) return rc;
-    if ((rc = ensure_pinned(h, (size_t)m * DIAG_WORDS))) return rc;
+    StepBufs B;
+    if (bufs) {
+        B = *bufs;
+    } else {
+        if ((rc = ensure_diag(h, m))) return rc;
+        if ((rc = ensure_pinned(h, (size_t)m * DIAG_WORDS))) return rc;
+        B = {h->d_diag, h->d_halt, true};
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ms) {
         HIPCHK(h, hipEventCreate(&e0));
@@ -400,19 +503,18 @@ int run_steps(dnsg_t *h, int m, dnsg_diag *diag, float *ms, SecTimer *T) {
             return DNSG_ERR_HIP;
         }
     }
-    std::vector<StepTop> tops;
-    tops.reserve(m);
+    const StepTop top0 = step_top(h);
     const long ss0 = h->ss;
-    hipError_t e = hipMemsetAsync(h->d_halt, 0, sizeof(int), h->stream);
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, h->stream, B.halt);
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess && ms) e = hipEventRecord(e0, h->stream);
     rc = e == hipSuccess ? DNSG_OK : DNSG_ERR_HIP;
     if (rc) h->err = std::string("step set-up: ") + hipGetErrorString(e);
+    h->halt_flag = B.halt;
     h->halt_armed = true;
-    for (int s = 0; s < m && !rc; s++) {
-        tops.push_back(step_top(h));
-        rc = do_step(h, h->d_diag + (size_t)s * DIAG_WORDS, T);
-    }
+    for (int s = 0; s < m && !rc; s++) rc = do_step(h, B.rec + (size_t)s * DIAG_WORDS, T);
     h->halt_armed = false;
+    h->halt_flag = h->d_halt;
     if (!rc && ms) {
         e = hipEventRecord(e1, h->stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -425,23 +527,33 @@ int run_steps(dnsg_t *h, int m, dnsg_diag *diag, float *ms, SecTimer *T) {
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (rc) return rc;
-    HIPCHK(h, hipMemcpyAsync(h->h_pinned, h->d_diag, sizeof(double) * DIAG_WORDS * m, hipMemcpyDeviceToHost,
-                             h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    for (int s = 0; s < m; s++) {
-        const double *rec = h->h_pinned + (size_t)s * DIAG_WORDS;
-        if (diag) fill_diag(h, rec, ss0 + s, diag + s);
-        if (isnan(rec[7])) {
-            restore_top(h, tops[s]);
-            h->ss = ss0 + s;
-            h->force = rec[1];
-            char buf[96];
-            snprintf(buf, sizeof buf, "NaN in wall shear stress at step %ld", ss0 + s);
-            h->err = buf;
-            return DNSG_ERR_NAN;
+    constexpr int CH = 64;  // records per copy through the stack buffer
+    double stage[CH * DIAG_WORDS];
+    double last_force = h->force;
+    for (int c0 = 0; c0 < m; c0 += CH) {
+        const int nc = B.pinned ? m : (m - c0 < CH ? m - c0 : CH);
+        double *hb = B.pinned ? h->h_pinned : stage;
+        HIPCHK(h, hipMemcpyAsync(hb, B.rec + (size_t)c0 * DIAG_WORDS, sizeof(double) * DIAG_WORDS * nc,
+                                 hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        for (int i = 0; i < nc; i++) {
+            const int s = c0 + i;
+            const double *rec = hb + (size_t)i * DIAG_WORDS;
+            if (diag) fill_diag(h, rec, ss0 + s, diag + s);
+            last_force = rec[1];
+            if (isnan(rec[7])) {
+                restore_top(h, top_after(h, top0, s));
+                h->ss = ss0 + s;
+                h->force = rec[1];
+                char buf[96];
+                snprintf(buf, sizeof buf, "NaN in wall shear stress at step %ld", ss0 + s);
+                h->err = buf;
+                return DNSG_ERR_NAN;
+            }
         }
+        if (B.pinned) break;
     }
-    h->force = h->h_pinned[(size_t)(m - 1) * DIAG_WORDS + 1];
+    h->force = last_force;
     return DNSG_OK;
 }
 
@@ -673,7 +785,7 @@ int dnsg_destroy(dnsg_t *h) {
     for (auto *g : gs)
         for (int b = 0; b < 2; b++)
             if (g->buf[b]) (void)hipFree(g->buf[b]);
-    for (int l = 0; l < 3; l++) {
+    for (int l = 0; l < 4; l++) {
         if (h->lev_lo[l]) (void)hipFree(h->lev_lo[l]);
         if (h->lev_up[l]) (void)hipFree(h->lev_up[l]);
     }
@@ -714,6 +826,12 @@ int dnsg::create_ctx(const dnsg_params *p, hipStream_t ext, dnsg_t **out) {
         HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     }
     h->elem = p->precision == DNSG_PREC_FP64 ? 8 : (p->precision == DNSG_PREC_FP32 ? 4 : 2);
+    {
+        // gratio 2: both band substeps in one pass (k_band2.hip); DNSG_BAND2=0 in the
+        // environment selects the one-launch-per-substep path instead (same results)
+        const char *e = getenv("DNSG_BAND2");
+        h->band2 = h->d.GR == 2 && (e && e[0] == '1');
+    }
     const dnsg_dims &d = h->d;
     h->coarse.nx = d.Xc; h->coarse.ny = d.Yc; h->coarse.nz = d.zdc;
     h->lower.nx = d.Xf; h->lower.ny = d.Yf; h->lower.nz = d.zdf - 1;
@@ -726,7 +844,7 @@ int dnsg::create_ctx(const dnsg_params *p, hipStream_t ext, dnsg_t **out) {
             HIPCHK(h, hipMemsetAsync(g->buf[b], 0, bytes, h->stream));
         }
     const size_t np = (size_t)d.Xc * d.Yc * NVP;
-    for (int l = 0; l < 3; l++) {
+    for (int l = 0; l < 4; l++) {
         HIPCHK(h, hipMalloc(&h->lev_lo[l], np * 8));
         HIPCHK(h, hipMalloc(&h->lev_up[l], np * 8));
         HIPCHK(h, hipMemsetAsync(h->lev_lo[l], 0, np * 8, h->stream));
@@ -741,7 +859,7 @@ int dnsg::create_ctx(const dnsg_params *p, hipStream_t ext, dnsg_t **out) {
     HIPCHK(h, hipMalloc(&h->part, h->part_cap * 8));
     {
         const size_t nred =
-            ((size_t)d.Yc * d.zdc * cs_chunks(d.Xc) + 2 * (size_t)d.Yf * (d.zdf - 1) * cs_chunks(d.Xf)) * RED_W;
+            ((size_t)d.Yc * d.zdc * cs_chunks(d.Xc) + 2 * (size_t)d.Yf * (d.zdf - 1) * band_cpr_max(h)) * RED_W;
         HIPCHK(h, hipMalloc(&h->red, nred * 8));
         HIPCHK(h, hipMemsetAsync(h->red, 0, nred * 8, h->stream));
         HIPCHK(h, hipMalloc(&h->seg, enc_seg_doubles(h) * 8));
@@ -753,6 +871,7 @@ int dnsg::create_ctx(const dnsg_params *p, hipStream_t ext, dnsg_t **out) {
     HIPCHK(h, hipMemsetAsync(h->d_force, 0, 4 * 8, h->stream));
     HIPCHK(h, hipMalloc(&h->d_halt, sizeof(int)));
     HIPCHK(h, hipMemsetAsync(h->d_halt, 0, sizeof(int), h->stream));
+    h->halt_flag = h->d_halt;
     const size_t nred = 4 * ((size_t)d.zdc + 2 * (d.zdf - 1)) + 2 * 9 * (size_t)d.NZ + 64;
     HIPCHK(h, hipMalloc(&h->d_red, nred * 8));
     HIPCHK(h, hipMemsetAsync(h->d_red, 0, nred * 8, h->stream));
@@ -1035,6 +1154,12 @@ int dnsg_run_timed_sections(dnsg_t *h, int n, dnsg_diag *diag, double *elapsed_m
     if (rc) return rc;
     if (sec_ms && (rc = sec_average(h, T, sec_ms))) return rc;
     if (elapsed_ms) *elapsed_ms = ms;
+    return DNSG_OK;
+}
+
+int dnsg_band_launch_substeps(const dnsg_t *h, int *substeps) {
+    if (!h || !substeps) return DNSG_ERR_ARG;
+    *substeps = h->band2 ? 2 : 1;
     return DNSG_OK;
 }
 

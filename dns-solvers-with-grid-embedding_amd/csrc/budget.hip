@@ -578,3 +578,65 @@ done:
     if (dpl) (void)hipFree(dpl);
     return rc;
 }
+
+// ---- time averages of the per-snapshot profiles ----------------------------------------
+// The reference averages the per-snapshot files afterwards with its serial
+// post-processor: time_stat.c:132-210 (turb-fld-2d: the mean of every profile except
+// UTIN, VTIN, WTIN and DTIN, whose squares it averages), vorticity.c:73-98 (the
+// squares of the three vorticity rms) and Ebudget.c:184-300 (the mean of every budget
+// term).  Here the same sums are folded at each snapshot, in FP64, in the handle: the
+// per-snapshot pass over the lattices is the device's work; the fold adds 52 profiles
+// of NZ values per snapshot.
+namespace {
+constexpr int BUD_ROWS = 19 + 3 + 30;
+// turb rows averaged as squares: UTIN VTIN WTIN (3..5) and DTIN (8); every vort row
+inline bool bud_squared(int row) { return row == 3 || row == 4 || row == 5 || row == 8 || (row >= 19 && row < 22); }
+}  // namespace
+
+extern "C" int dnsg_budget2d_accumulate(dnsg_t *h, double *turb, double *vort, double *ke) {
+    if (!h || !h->stream) return DNSG_ERR_ARG;
+    const int NZ = h->d.NZ;
+    std::vector<double> prof((size_t)BUD_ROWS * NZ);
+    double *t = prof.data(), *v = t + 19 * (size_t)NZ, *k = v + 3 * (size_t)NZ;
+    int rc = dnsg_budget2d(h, t, v, k, nullptr, nullptr);
+    if (rc) return rc;
+    if (h->bud_acc.size() != prof.size()) {
+        h->bud_acc.assign(prof.size(), 0.);
+        h->bud_count = 0;
+    }
+    for (int r = 0; r < BUD_ROWS; r++)
+        for (int z = 0; z < NZ; z++) {
+            const double x = prof[(size_t)r * NZ + z];
+            h->bud_acc[(size_t)r * NZ + z] += bud_squared(r) ? x * x : x;
+        }
+    h->bud_count++;
+    if (turb) memcpy(turb, t, 19 * (size_t)NZ * 8);
+    if (vort) memcpy(vort, v, 3 * (size_t)NZ * 8);
+    if (ke) memcpy(ke, k, 30 * (size_t)NZ * 8);
+    return DNSG_OK;
+}
+
+extern "C" int dnsg_budget2d_average(dnsg_t *h, double *turb, double *vort, double *ke, long *count) {
+    if (!h || !turb || !vort || !ke) return DNSG_ERR_ARG;
+    const int NZ = h->d.NZ;
+    if (count) *count = h->bud_count;
+    if (h->bud_count == 0) {
+        h->err = "no snapshot folded: call dnsg_budget2d_accumulate first";
+        return DNSG_ERR_STATE;
+    }
+    double *dst[3] = {turb, vort, ke};
+    const int rows[3] = {19, 3, 30};
+    int r0 = 0;
+    for (int g = 0; g < 3; g++) {
+        for (size_t i = 0; i < (size_t)rows[g] * NZ; i++) dst[g][i] = h->bud_acc[(size_t)r0 * NZ + i] / h->bud_count;
+        r0 += rows[g];
+    }
+    return DNSG_OK;
+}
+
+extern "C" int dnsg_budget2d_reset(dnsg_t *h) {
+    if (!h) return DNSG_ERR_ARG;
+    h->bud_acc.clear();
+    h->bud_count = 0;
+    return DNSG_OK;
+}

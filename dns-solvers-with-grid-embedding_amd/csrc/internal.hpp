@@ -19,6 +19,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/dnsg.h"
 
@@ -102,6 +103,11 @@ struct Grid {
     void swap() { cur ^= 1; }
 };
 
+// Columns per tile of the two-substep band kernel (k_band2.hip).  When a step runs
+// it, the bands' fused-reduction partials (LatArgs::red, ::dsum) are per (layer, row,
+// B2_TX columns) instead of per CS_THREADS columns; dnsg_ctx::band_cols says which.
+constexpr int B2_TX = 64;
+
 // number of doubles in one diag record written by the reduction finalisers
 constexpr int DIAG_WORDS = 9;
 // per-row partial sums kept for one reduction pass
@@ -121,9 +127,16 @@ struct dnsg_ctx {
 
     dnsg::Grid coarse, lower, upper;
     // interface time levels [23][Yc][Xc], newest first, rotated by pointer swaps;
-    // FP64 in the plain tier, FP32 (slot 0 = rho - 1) in the shifted tiers
-    void *lev_lo[3] = {nullptr, nullptr, nullptr};
-    void *lev_up[3] = {nullptr, nullptr, nullptr};
+    // FP64 in the plain tier, FP32 (slot 0 = rho - 1) in the shifted tiers.  Slot 3
+    // is a spare: the sampling of a step writes the spare, which the rotation makes
+    // the newest level, so the three levels at the top of a step stay intact until
+    // the step's wall-stress check has passed
+    void *lev_lo[4] = {nullptr, nullptr, nullptr, nullptr};
+    void *lev_up[4] = {nullptr, nullptr, nullptr, nullptr};
+    // the steps use the two-substep band kernel (gratio 2); band_cols: columns per
+    // fused-reduction partial in the bands' partials of the last launch that wrote them
+    bool band2 = false;
+    int band_cols = dnsg::CS_THREADS;
     int8_t *wall_lo = nullptr, *wall_up = nullptr;     // [Yf][Xf]
 
     // reductions
@@ -148,6 +161,9 @@ struct dnsg_ctx {
     // running statistics accumulators [14][NZ] + count
     double *d_acc = nullptr;
     long acc_count = 0;
+    // time sums of the budget profiles (dnsg_budget2d_accumulate): [19 + 3 + 30][NZ]
+    std::vector<double> bud_acc;
+    long bud_count = 0;
 
     // host copies of the band wall layers (never touched by the step) and their
     // mass contribution, for checkpoints and the mass check
@@ -162,8 +178,9 @@ struct dnsg_ctx {
     // halt_armed (inside dnsg_step / dnsg_run_timed*) every state-changing launch of
     // the later kernels is a no-op, so the state stays that of the failing step's top
     int *d_halt = nullptr;
+    int *halt_flag = nullptr;  // the flag of the queued steps: d_halt, or the device's static scratch
     bool halt_armed = false;
-    const int *halt() const { return halt_armed ? d_halt : nullptr; }
+    const int *halt() const { return halt_armed ? halt_flag : nullptr; }
 
     double force = 0.;  // host mirror of V.force[0]
     long ss = 0;        // next step index of the time loop
@@ -171,6 +188,11 @@ struct dnsg_ctx {
 };
 
 namespace dnsg {
+
+// fused-reduction partials per (layer, row) of a band: in the current layout, and
+// the most any layout needs (the buffers are sized for it)
+inline int band_cpr(const dnsg_ctx *h) { return (h->d.Xf + h->band_cols - 1) / h->band_cols; }
+inline int band_cpr_max(const dnsg_ctx *h) { return (h->d.Xf + B2_TX - 1) / B2_TX; }
 
 // handle set-up and the per-step argument builders shared by the single-solve
 // step and the serving queue
@@ -184,6 +206,12 @@ void rotate_levels(dnsg_t *h);
 
 // launchers (stream-ordered, no host synchronisation)
 hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, int prec, const int *halt = nullptr);
+// both gratio-2 band substeps of one coarse step in one pass (k_band2.hip): bands[0]
+// the lower band, bands[1] the upper band, as band_args builds them; the first fill
+// must already be in their nxt lattices' interface layers; red / dsum in the B2_TX
+// partial layout
+hipError_t launch_band2(hipStream_t s, const LatArgs *bands, int prec, const int *halt = nullptr);
+int band2_tiles(int nx, int ny);
 hipError_t launch_init_lattice(hipStream_t s, dnsg_ctx *h, int grid, void *dst);
 // gl = 1: the host array carries one ghost x-plane and y-row (the reference's in-memory
 // layout); downloads write them as periodic images

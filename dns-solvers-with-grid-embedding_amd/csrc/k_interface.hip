@@ -383,7 +383,7 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
                 if (hok[k]) { a[k][0] = Ln[vo + hoff[k]]; a[k][1] = Lp[vo + hoff[k]]; a[k][2] = Lpp[vo + hoff[k]]; }
         }
         __syncthreads();
-        // 2. d/dy on corner rows for every halo column; d/dx at the corners (derivative.c:118-119)
+        // 2. d/dy on corner rows for every halo column; d/dx at the corners (derivative.c:41-42, 63-64)
         P(*t0)[HX] = s0[sb];
         for (int p = tid; p < (TY + 1) * HX; p += NT) {
             const int rr = p / HX, hx = p % HX, hy = rr + 2;
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
         }
         __syncthreads();
         // 3. bicubic patch of this cell (bcucof_opt.c:47-61) at its fine columns;
-        //    d2/dxdy = d/dx of d/dy (derivative.c:139) at the cell's corners, from
+        //    d2/dxdy = d/dx of d/dy (derivative.c:84) at the cell's corners, from
         //    the d/dy of columns cx .. cx+5 on its two corner rows
         P xv[16];
         P dyr[2][6];
@@ -552,14 +552,18 @@ hipError_t launch_interface_many(hipStream_t s, dnsg_ctx *const *hs, int n, int 
         return w == 0. || std::fabs(std::frexp(w, &e)) == 0.5;
     };
     H.exact = pow2_or_zero(H.h00) && pow2_or_zero(H.h10) && pow2_or_zero(H.h01) && pow2_or_zero(H.h11);
-    std::vector<IfaceItem> v(2 * (size_t)n);
+    // one handle (the single-solve step): its two items on the stack, so the step
+    // allocates no host memory; a batch of several (the serving queue): a vector
+    IfaceItem one[2];
+    std::vector<IfaceItem> many(n > 1 ? 2 * (size_t)n : 0);
+    IfaceItem *v = n > 1 ? many.data() : one;
     for (int i = 0; i < n; i++) iface_pair(hs[i], &v[2 * (size_t)i]);
     // one handle: its own halt flag; a batch of several (the serving queue): none
     const int *halt = n == 1 ? hs[0]->halt() : nullptr;
-    if (prec == DNSG_PREC_FP64) return iface_items<double, double>(s, GR, v.data(), 2 * n, H, halt);
+    if (prec == DNSG_PREC_FP64) return iface_items<double, double>(s, GR, v, 2 * n, H, halt);
     HermiteW<float> Hf = {(float)H.h00, (float)H.h10, (float)H.h01, (float)H.h11, H.exact};
-    if (prec == DNSG_PREC_FP32) return iface_items<float, float>(s, GR, v.data(), 2 * n, Hf, halt);
-    return iface_items<__half, float>(s, GR, v.data(), 2 * n, Hf, halt);
+    if (prec == DNSG_PREC_FP32) return iface_items<float, float>(s, GR, v, 2 * n, Hf, halt);
+    return iface_items<__half, float>(s, GR, v, 2 * n, Hf, halt);
 }
 
 hipError_t launch_interface(hipStream_t s, dnsg_ctx *h, int iter) { return launch_interface_many(s, &h, 1, iter); }
@@ -837,7 +841,9 @@ hipError_t launch_ftcd_many(hipStream_t s, dnsg_ctx *const *hs, int n, double *p
     if (n < 1) return hipSuccess;
     for (int i = 0; i < n; i++)  // the filtered injection runs per handle (the queue takes none)
         if (hs[i]->p.fltr) return n == 1 ? launch_fltr(s, hs[0], psum) : hipErrorInvalidValue;
-    std::vector<FtcdItem> v(n);
+    FtcdItem one[1];  // one handle: on the stack, as in launch_interface_many
+    std::vector<FtcdItem> many(n > 1 ? n : 0);
+    FtcdItem *v = n > 1 ? many.data() : one;
     for (int i = 0; i < n; i++) {
         const dnsg_ctx *h = hs[i];
         const int zdf = h->d.zdf, GR = h->d.GR;
@@ -851,7 +857,7 @@ hipError_t launch_ftcd_many(hipStream_t s, dnsg_ctx *const *hs, int n, double *p
     for (int i0 = 0; i0 < n; i0 += BATCH_MAX) {
         Batch<FtcdItem> B;
         B.halt = n == 1 ? hs[0]->halt() : nullptr;
-        const int blocks = fill_batch(B, v.data() + i0, std::min(BATCH_MAX, n - i0),
+        const int blocks = fill_batch(B, v + i0, std::min(BATCH_MAX, n - i0),
                                       [](const FtcdItem &a) { return 2 * ((a.Xc * a.Yc + 255) / 256); });
         if (prec == DNSG_PREC_FP64)
             hipLaunchKernelGGL(k_ftcd<double>, dim3(blocks), dim3(256), 0, s, B);

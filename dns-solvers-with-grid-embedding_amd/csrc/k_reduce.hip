@@ -1,5 +1,5 @@
 // k_reduce.hip -- FP64 reductions of the time step: bulk flux and density with
-// the flux controller (encalc2.c:17-147), wall shear stress (wshearstr.c:15-169),
+// the flux controller (encalc2.c:17-147), wall shear stress (wshearstr.c:15-168),
 // the global mass check (main:1193-1250) and the plane statistics
 // (statistics.c:15-300).
 //
@@ -231,7 +231,7 @@ static hipError_t enc_two_stage(hipStream_t s, const EncArgs &A, double *seg) {
 
 size_t enc_seg_doubles(const dnsg_ctx *h) {
     const size_t total = (size_t)h->d.Yc * h->d.zdc * cs_chunks(h->d.Xc) +
-                         2 * (size_t)h->d.Yf * (h->d.zdf - 1) * cs_chunks(h->d.Xf) + (size_t)ftcd_blocks(h);
+                         2 * (size_t)h->d.Yf * (h->d.zdf - 1) * band_cpr_max(h) + (size_t)ftcd_blocks(h);
     return 4 * (total / ENC_SEG + 2);
 }
 
@@ -245,7 +245,7 @@ size_t enc_seg_doubles(const dnsg_ctx *h) {
 
 size_t dsum_doubles(const dnsg_ctx *h) {
     return 2 * ((size_t)h->d.Yc * h->d.zdc * cs_chunks(h->d.Xc) +
-                2 * (size_t)h->d.Yf * (h->d.zdf - 1) * cs_chunks(h->d.Xf));
+                2 * (size_t)h->d.Yf * (h->d.zdf - 1) * band_cpr_max(h));
 }
 
 void set_dest_weights(const dnsg_ctx *h, int grid, LatArgs *L) {
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(RB) void k_flux_fin(FluxNextArgs A, const double *s
 hipError_t launch_flux_next(hipStream_t s, dnsg_ctx *h) {
     FluxNextArgs A;
     A.nc = h->d.Yc * h->d.zdc * cs_chunks(h->d.Xc);
-    A.nl = A.nu = h->d.Yf * (h->d.zdf - 1) * cs_chunks(h->d.Xf);
+    A.nl = A.nu = h->d.Yf * (h->d.zdf - 1) * band_cpr(h);
     A.nf = ftcd_blocks(h);
     A.dc = h->dsum;
     A.dl = A.dc + 2 * (size_t)A.nc;
@@ -330,7 +330,7 @@ static EncArgs enc_args(dnsg_ctx *h, const double *pc, const double *pl, const d
     EncArgs A;
     A.pc = pc; A.pl = pl; A.pu = pu; A.st = st;
     A.cpr_c = fused ? cs_chunks(h->coarse.nx) : 1;
-    A.cpr_f = fused ? cs_chunks(h->lower.nx) : 1;
+    A.cpr_f = fused ? band_cpr(h) : 1;
     A.Yc = h->d.Yc; A.zdc = h->d.zdc; A.Yf = h->d.Yf; A.zds = h->lower.nz; A.GR = h->d.GR; A.zdf = h->d.zdf;
     A.NZs = h->d.NZs; A.Xf = h->d.Xf; A.force_mode = h->p.force_mode; A.ub = h->p.ub; A.g_const = h->p.g_const;
     A.force = h->d_force; A.diag = diag_rec;
@@ -340,7 +340,7 @@ static EncArgs enc_args(dnsg_ctx *h, const double *pc, const double *pl, const d
 // encalc2 from the row sums the collide-stream kernels wrote (RED_W per row)
 hipError_t launch_encalc2_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec) {
     const double *pc = h->red, *pl = pc + (size_t)h->coarse.ny * h->coarse.nz * cs_chunks(h->coarse.nx) * RED_W;
-    const double *pu = pl + (size_t)h->lower.ny * h->lower.nz * cs_chunks(h->lower.nx) * RED_W;
+    const double *pu = pl + (size_t)h->lower.ny * h->lower.nz * band_cpr(h) * RED_W;
     EncArgs A = enc_args(h, pc, pl, pu, RED_W, true, diag_rec);
     return enc_two_stage(s, A, h->seg);
 }
@@ -373,7 +373,7 @@ hipError_t launch_flux_many(hipStream_t s, dnsg_ctx *const *hs, int n, double *o
             dnsg_ctx *h = hs[i0 + (i < B.n ? i : 0)];
             const double *pc = h->red;
             const double *pl = pc + (size_t)h->coarse.ny * h->coarse.nz * cs_chunks(h->coarse.nx) * RED_W;
-            const double *pu = pl + (size_t)h->lower.ny * h->lower.nz * cs_chunks(h->lower.nx) * RED_W;
+            const double *pu = pl + (size_t)h->lower.ny * h->lower.nz * band_cpr(h) * RED_W;
             B.it[i] = enc_args(h, pc, pl, pu, RED_W, true, out + 2 * (size_t)slot[i0 + (i < B.n ? i : 0)]);
             B.first[i] = i < B.n ? i : B.n;
         }
@@ -470,15 +470,15 @@ __global__ __launch_bounds__(RB) void k_fin_wshear_fused(WsArgs A, const double 
 
 __device__ void wshear_finish(const WsArgs &A, const double *v) {
     {
-        const double mu = (1. / 6.) * (2. / A.omega_f - 1);              // wshearstr.c:181
+        const double mu = (1. / 6.) * (2. / A.omega_f - 1);              // wshearstr.c:35
         const double area = ((double)((A.Xf + 1 - 1.) * (A.Yf + 1 - 1.)));
-        double davg = v[1] / area, stdown = v[0] / area;                   // :218-219
-        stdown *= mu * davg;                                               // :227
-        double davgu = v[3] / area, stup = v[2] / area;                    // :266-267
+        double davg = v[1] / area, stdown = v[0] / area;                   // :72-73
+        stdown *= mu * davg;                                               // :81
+        double davgu = v[3] / area, stup = v[2] / area;                    // :120-121
         double pgrad = A.force[0];
-        pgrad = pgrad * (A.NZs - 2.) / (2. * A.GR);                        // :278
-        stup *= mu * davgu;                                                // :280
-        const double stress = 0.5 * (stup + stdown);                       // :284
+        pgrad = pgrad * (A.NZs - 2.) / (2. * A.GR);                        // :130-132
+        stup *= mu * davgu;                                                // :134
+        const double stress = 0.5 * (stup + stdown);                       // :138
         if (A.diag) { A.diag[5] = stup; A.diag[6] = stdown; A.diag[7] = stress; A.diag[8] = pgrad; }
         if (A.out) { A.out[0] = stup; A.out[1] = stdown; A.out[2] = stress; A.out[3] = pgrad; }
         if (A.halt && isnan(stress)) A.halt[0] = 1;
@@ -486,13 +486,13 @@ __device__ void wshear_finish(const WsArgs &A, const double *v) {
 }
 
 hipError_t launch_wshear_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec, const double *enc_rec) {
-    const int cpr = cs_chunks(h->lower.nx);
+    const int cpr = band_cpr(h);
     const double *pl = h->red + (size_t)h->coarse.ny * h->coarse.nz * cs_chunks(h->coarse.nx) * RED_W;
     const double *pu = pl + (size_t)h->lower.ny * h->lower.nz * cpr * RED_W;
     WsArgs A;
     A.part = nullptr; A.Xf = h->d.Xf; A.Yf = h->d.Yf; A.NZs = h->d.NZs; A.GR = h->d.GR; A.omega_f = h->d.omega_f;
     A.force = h->d_force; A.diag = diag_rec; A.out = nullptr; A.enc = enc_rec;
-    A.halt = h->halt_armed ? h->d_halt : nullptr;
+    A.halt = h->halt_armed ? h->halt_flag : nullptr;
     hipLaunchKernelGGL(k_fin_wshear_fused, dim3(1), dim3(RB), 0, s, A, pl, pu, h->lower.nz, cpr);
     return hipGetLastError();
 }

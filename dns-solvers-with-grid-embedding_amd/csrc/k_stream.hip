@@ -16,35 +16,9 @@
 //                 otherwise              -> specular reflection  f(x + e_xy, mirror q)
 #include "internal.hpp"
 #include "lattice.hpp"
+#include "stream_common.hpp"
 
 namespace dnsg {
-
-template <class S>
-struct ComputeOf {
-    using T = float;
-};
-template <>
-struct ComputeOf<double> {
-    using T = double;
-};
-
-__device__ __forceinline__ void collide_any(double *f, double G, double om) { collide64(f, G, om); }
-__device__ __forceinline__ void collide_any(float *f, float G, float om) { collide32(f, G, om); }
-
-__device__ __forceinline__ double wave_sum64(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs; remap the linear block
-// id so that each XCD gets one contiguous range of (row, chunk) pairs.  The
-// neighbouring chunks of a row, whose e_x-shifted writes share cache lines,
-// then meet in the same L2.
-__device__ __forceinline__ int xcd_remap(int b, int nb) {
-    const int per = nb >> 3;
-    return (b < (per << 3)) ? (b & 7) * per + (b >> 3) : b;
-}
 
 // Per storage type: use the aligned-store form (bit 0 FP64, 1 FP32, 2 FP16) and,
 // in it, non-temporal stores.  Measured on the Re_tau=180 lattices: the aligned
@@ -78,48 +52,12 @@ __device__ __forceinline__ auto load_cs(const S *p) {
     else return load_s(p);
 }
 
-// fused-reduction sums of one thread's nodes: pre-collision moments (rho, mom, u)
-// and the destination-weighted sums of what it writes (dm, dd)
-struct RowSums {
-    double rho = 0., mom = 0., u = 0., dm = 0., dd = 0.;
-    template <class S, class T>
-    __device__ __forceinline__ void add(const T *f) {
-        double d, m;
-        if constexpr (sizeof(S) == 8) {
-            m = first_moment<0>(f);
-            d = zeroth_moment(f);
-        } else {
-            m = (double)first_moment<0>(f);
-            d = 1. + (double)zeroth_moment(f);
-        }
-        rho += d;
-        mom += m;
-        u += m / d;
-    }
-};
-
-// encalc2 weight of a destination layer (LatArgs::dsum)
-__device__ __forceinline__ double dest_w(const LatArgs &L, int zt) {
-    return (zt < L.klo || zt > L.khi) ? 0. : (zt == L.khalf ? 0.5 * L.kscale : L.kscale);
-}
-
-// add a store of direction d (e_x = ex) and f-value fv into a layer of weight W
-__device__ __forceinline__ void add_write(RowSums &rs, double W, int ex, double fv) {
-    const double a = W * fv;
-    rs.dd += a;
-    if (ex > 0) rs.dm += a;
-    else if (ex < 0) rs.dm -= a;
-}
-
 // Scatter form for the node at x of row (y, z): collide, then write each
 // direction to its target (x + e_x, y + e_y, z + e_z) with the wall rules.
 template <class S, bool DS, class T>
 __device__ __forceinline__ void cs_scatter(const LatArgs &L, const S *__restrict__ cur, S *__restrict__ nxt, int row,
                                            int x, T G, T om, bool red, RowSums &rs) {
-    const int nx = L.nx, ny = L.ny, nz = L.nz;
-    const int y = row % ny, z = row / ny;
-    const bool lo_edge = (z == 0), hi_edge = (z == nz - 1);
-    const int ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
+    const int nx = L.nx;
     T f[Q];
     unroll<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
@@ -127,37 +65,7 @@ __device__ __forceinline__ void cs_scatter(const LatArgs &L, const S *__restrict
     });
     if (red) rs.add<S>(f);
     collide_any(f, G, om);
-    const bool ds = DS && L.dsum != nullptr;  // DS: compiled in only where a launch asks for it
-    const double wz[3] = {ds ? dest_w(L, z - 1) : 0., ds ? dest_w(L, z) : 0., ds ? dest_w(L, z + 1) : 0.};
-    const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
-    unroll<0, Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        const int xt = (EX[q] > 0) ? xp : (EX[q] < 0 ? xm : x);
-        const int yt = (EY[q] > 0) ? yp : (EY[q] < 0 ? ym : y);
-        if constexpr (EZ[q] != 0) {
-            const bool at_end = (EZ[q] < 0) ? lo_edge : hi_edge;
-            if (at_end) {
-                const int kind = (EZ[q] < 0) ? L.lo_kind : L.hi_kind;
-                if (kind == BOUND_WALL) {
-                    const int8_t *wc = (EZ[q] < 0) ? L.wlo : L.whi;
-                    const int sld = wc[(size_t)yt * nx + xt];
-                    const int sld1 = wc[(size_t)y * nx + x];
-                    if (sld == -1 && sld1 == -1) {
-                        store_s(nxt + lat_at(OPP[q], (size_t)row, x, nx), f[q]);
-                        if (ds) add_write(rs, wz[1], -EX[q], read_back<S, q>(f[q]));
-                    } else {
-                        store_s(nxt + lat_at(MIR[q], (size_t)z * ny + yt, xt, nx), f[q]);
-                        if (ds) add_write(rs, wz[1], EX[q], read_back<S, q>(f[q]));
-                    }
-                }
-                return;
-            }
-        }
-        const int zt = z + EZ[q];
-        if ((zt == 0 && L.skip_lo) || (zt == nz - 1 && L.skip_hi)) return;
-        store_s(nxt + lat_at(q, (size_t)zt * ny + yt, xt, nx), f[q]);
-        if (ds) add_write(rs, wz[EZ[q] + 1], EX[q], read_back<S, q>(f[q]));
-    });
+    scatter_push<S, DS>(L, nxt, row, x, f, rs);
 }
 
 // Aligned form for chunk x0 .. x0 + CS_THREADS - 1 of row (y, z) (nx a multiple of

@@ -19,6 +19,7 @@
 // U0 = G H^2 / (12 nu_c); equal slip b on both walls raises it to
 // U0 (1 + 6 b / H), so b = (H/6)(U/U0 - 1).
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -36,6 +37,7 @@ namespace {
 
 struct Solve {
     uint64_t id = 0;
+    uint64_t serial = 0;  // admission number, unique over the queue's life
     dnsg_params p;
     dnsg_dims d;
     size_t bytes = 0;
@@ -45,19 +47,24 @@ struct Solve {
     int slot = -1;
 };
 
-// device bytes of one resident solve (what create_ctx allocates)
+// Device bytes of one resident solve, estimated before it is created: what create_ctx
+// allocates, each buffer of 1 MiB or more rounded up to the runtime's 2 MiB allocation
+// granularity, plus 2 MiB for the small buffers' pool.  The queue replaces it by the
+// measured footprint when that is larger (admit).
+constexpr size_t GRAN = (size_t)2 << 20;
+inline size_t gran(size_t b) { return b >= ((size_t)1 << 20) ? (b + GRAN - 1) / GRAN * GRAN : b; }
 size_t solve_bytes(const dnsg_params *p, const dnsg_dims &d) {
     const size_t elem = p->precision == DNSG_PREC_FP64 ? 8 : (p->precision == DNSG_PREC_FP32 ? 4 : 2);
     const size_t nc = (size_t)d.Xc * d.Yc * d.zdc, nb = (size_t)d.Xf * d.Yf * (d.zdf - 1);
-    size_t b = 2 * (nc + 2 * nb) * 19 * elem;               // A/B lattices
-    b += 6 * (size_t)d.Xc * d.Yc * 23 * 8;                  // interface levels
-    b += 2 * (size_t)d.Xf * d.Yf;                           // wall codes
+    size_t b = 2 * gran(nc * 19 * elem) + 4 * gran(nb * 19 * elem);  // A/B lattices
+    b += 8 * gran((size_t)d.Xc * d.Yc * 23 * 8);            // interface levels (3 + a spare per side)
+    b += 2 * gran((size_t)d.Xf * d.Yf);                     // wall codes
     const size_t rows = (size_t)d.Yc * d.zdc + 2 * (size_t)d.Yf * (d.zdf - 1);
     b += rows * 5 * 8 + (size_t)d.Yf * 4 * 8;               // explicit-pass partials
-    b += ((size_t)d.Yc * d.zdc * cs_chunks(d.Xc) + 2 * (size_t)d.Yf * (d.zdf - 1) * cs_chunks(d.Xf)) * RED_W * 8 * 2;
+    b += ((size_t)d.Yc * d.zdc * cs_chunks(d.Xc) + 2 * (size_t)d.Yf * (d.zdf - 1) * (size_t)((d.Xf + B2_TX - 1) / B2_TX)) * RED_W * 8 * 2;
     b += (4 * ((size_t)d.zdc + 2 * (d.zdf - 1)) + 2 * 9 * (size_t)d.NZ + 64) * 8 + 14 * (size_t)d.NZ * 8;
     b += 64 * 9 * 8 + 4 * 8;
-    return b + (1u << 20);  // allocation granularity
+    return b + GRAN;  // the small buffers' pool
 }
 
 }  // namespace
@@ -72,6 +79,17 @@ struct dnsg_queue {
     size_t budget = 0, resident = 0;
     double *d_u = nullptr, *h_u = nullptr;  // 2 doubles per slot: <rho u_x>, <rho>
     std::chrono::steady_clock::time_point t0;
+    uint64_t admitted = 0;
+    // One check interval of the resident set as a HIP graph: captured once the set
+    // has stayed the same for a whole interval, replayed while it stays so.  Every
+    // pointer the launches take returns to its value after an interval (the buffer
+    // swaps and the four-level rotation have periods dividing 4, and intervals are
+    // multiples of 4 steps), so one capture serves every later interval of that set.
+    bool use_graph = true;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uint64_t> gset;     // serials of the set the graph belongs to
+    std::vector<uint64_t> last_set;  // serials of the set of the previous interval
+    long graph_launches = 0;
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
 
@@ -122,8 +140,15 @@ int admit(dnsg_queue *q) {
             ++it;
             continue;
         }
+        // the device bytes the solve actually takes (the runtime rounds and pools
+        // allocations), measured around its creation; accounted as the larger of that
+        // and the estimate the admission used
+        size_t fr0 = 0, fr1 = 0, tot = 0;
+        const bool measured = hipMemGetInfo(&fr0, &tot) == hipSuccess;
         int rc = create_ctx(&s->p, q->stream, &s->h);
         if (!rc) rc = dnsg_init_state(s->h);
+        if (!rc && measured && hipMemGetInfo(&fr1, &tot) == hipSuccess && fr0 > fr1)
+            s->bytes = std::max(s->bytes, fr0 - fr1);
         if (rc) {
             q->err = std::string("admitting solve: ") + (s->h ? dnsg_last_error(s->h) : "");
             if (s->h) dnsg_destroy(s->h);
@@ -135,6 +160,7 @@ int admit(dnsg_queue *q) {
             continue;
         }
         s->t_admit = q->now();
+        s->serial = ++q->admitted;
         q->resident += s->bytes;
         q->active.push_back(s);
         it = q->pending.erase(it);
@@ -251,6 +277,59 @@ int check(dnsg_queue *q) {
     return DNSG_OK;
 }
 
+void drop_graph(dnsg_queue *q) {
+    if (q->gexec) (void)hipGraphExecDestroy(q->gexec);
+    q->gexec = nullptr;
+    q->gset.clear();
+}
+
+// one check interval of the resident set: check_every ticks, the last with the flux
+// sums; eager launches, or the interval's graph (captured here when the set was
+// already resident for the whole previous interval)
+int interval(dnsg_queue *q) {
+    const int ce = q->o.check_every;
+    std::vector<uint64_t> set;
+    set.reserve(q->active.size());
+    for (const Solve *s : q->active) set.push_back(s->serial);
+    const bool stable = set == q->last_set;
+    q->last_set = set;
+    if (q->gexec && set != q->gset) drop_graph(q);
+    if (q->gexec) {
+        QCHK(q, hipGraphLaunch(q->gexec, q->stream));
+        q->graph_launches++;
+        for (Solve *s : q->active) s->h->ss += ce;
+        return DNSG_OK;
+    }
+    if (!(q->use_graph && stable && ce % 4 == 0)) {
+        int rc = DNSG_OK;
+        for (int k = 0; k < ce && !rc; k++) rc = tick(q, k == ce - 1);
+        return rc;
+    }
+    // capture: the ticks record their launches (and advance the host pointers through
+    // a whole period) instead of running them; then the graph runs the interval
+    QCHK(q, hipStreamBeginCapture(q->stream, hipStreamCaptureModeThreadLocal));
+    int rc = DNSG_OK;
+    for (int k = 0; k < ce && !rc; k++) rc = tick(q, k == ce - 1);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(q->stream, &g);
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (!rc) q->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+        return rc ? rc : DNSG_ERR_HIP;
+    }
+    const hipError_t ei = hipGraphInstantiate(&q->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        q->gexec = nullptr;
+        q->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(ei);
+        return DNSG_ERR_HIP;
+    }
+    q->gset = set;
+    QCHK(q, hipGraphLaunch(q->gexec, q->stream));
+    q->graph_launches++;
+    return DNSG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -280,6 +359,11 @@ int dnsg_queue_create(const dnsg_queue_opts *o, dnsg_queue_t **out) {
         q->err = "invalid queue options";
         return DNSG_ERR_ARG;
     }
+    {
+        // DNSG_QUEUE_GRAPH=0 in the environment runs every interval with eager launches
+        const char *e = getenv("DNSG_QUEUE_GRAPH");
+        q->use_graph = !(e && e[0] == '0');
+    }
     QCHK(q, hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking));
     QCHK(q, hipMalloc(&q->d_u, 2 * sizeof(double) * DNSG_QUEUE_MAX_ACTIVE));
     QCHK(q, hipHostMalloc((void **)&q->h_u, 2 * sizeof(double) * DNSG_QUEUE_MAX_ACTIVE, hipHostMallocDefault));
@@ -297,6 +381,7 @@ int dnsg_queue_create(const dnsg_queue_opts *o, dnsg_queue_t **out) {
 int dnsg_queue_destroy(dnsg_queue_t *q) {
     if (!q) return DNSG_OK;
     if (q->stream) (void)hipStreamSynchronize(q->stream);
+    drop_graph(q);
     for (Solve *s : q->active) {
         dnsg_destroy(s->h);
         delete s;
@@ -353,7 +438,7 @@ int dnsg_queue_advance(dnsg_queue_t *q, int steps, dnsg_slip_result *done, int c
     int rc = admit(q);
     const int ce = q->o.check_every;
     for (int s = 0; !rc && s < steps && !q->active.empty(); s += ce) {
-        for (int k = 0; k < ce && !rc; k++) rc = tick(q, k == ce - 1);
+        rc = interval(q);
         if (!rc) rc = check(q);
         if (!rc) rc = admit(q);
     }
@@ -362,6 +447,12 @@ int dnsg_queue_advance(dnsg_queue_t *q, int steps, dnsg_slip_result *done, int c
         q->finished.pop_front();
     }
     return rc;
+}
+
+int dnsg_queue_graph_launches(const dnsg_queue_t *q, long *launches) {
+    if (!q || !launches) return DNSG_ERR_ARG;
+    *launches = q->graph_launches;
+    return DNSG_OK;
 }
 
 int dnsg_queue_info(const dnsg_queue_t *q, int *pending, int *active, size_t *resident_bytes) {

@@ -62,7 +62,7 @@ enum dnsg_status {
     DNSG_OK        =  0,
     DNSG_ERR_ARG   = -1, /* invalid argument or parameter set                       */
     DNSG_ERR_HIP   = -2, /* HIP runtime error                                        */
-    DNSG_ERR_NAN   = -3, /* NaN in the wall shear stress (wshearstr.c:285-289)       */
+    DNSG_ERR_NAN   = -3, /* NaN in the wall shear stress (wshearstr.c:139-143)       */
     DNSG_ERR_OOM   = -4, /* device allocation failed                                 */
     DNSG_ERR_STATE = -5, /* call out of sequence                                     */
     DNSG_ERR_IO    = -6  /* file read/write failed (driver-side helpers)             */
@@ -132,10 +132,10 @@ typedef struct dnsg_diag {
     double momentum;      /* <rho u_x> weighted volume mean (encalc2.c:109)         */
     double density;       /* density.txt value (encalc2.c:142)                      */
     double flowrate;      /* flowrate.txt value (encalc2.c:138)                     */
-    double str_up;        /* str-up.txt (wshearstr.c:280)                           */
-    double str_down;      /* str-do.txt (wshearstr.c:227)                           */
-    double stress;        /* w-str.txt  (wshearstr.c:284)                           */
-    double pgrad;         /* p-grad.txt (wshearstr.c:278)                           */
+    double str_up;        /* str-up.txt (wshearstr.c:134, 149)                      */
+    double str_down;      /* str-do.txt (wshearstr.c:81, 154)                       */
+    double stress;        /* w-str.txt  (wshearstr.c:138, 159)                      */
+    double pgrad;         /* p-grad.txt (wshearstr.c:130-132, 164)                  */
 } dnsg_diag;
 
 typedef struct dnsg_ctx dnsg_t;
@@ -243,10 +243,17 @@ int dnsg_run_timed_sections(dnsg_t *h, int n, dnsg_diag *diag, double *elapsed_m
  * loop with MPI_Wtime, main:1252,1377): run n steps with HIP events between the
  * kernel groups on the handle's stream and return the average device time in ms
  * per launch of each group: ms[0] reductions (encalc2 + wshearstr),
- * ms[1] coarse collide-stream, ms[2] interface sampling, ms[3] band
- * collide-stream (both bands, one launch per substep), ms[4] interface fill
+ * ms[1] coarse collide-stream (with the bands' first substep unless
+ * dnsg_band_launch_substeps gives 2), ms[2] interface sampling, ms[3] band
+ * collide-stream (both bands: one launch per substep, or one launch for both
+ * substeps where dnsg_band_launch_substeps gives 2), ms[4] interface fill
  * (+ fine->coarse injection after the last substep). */
 int dnsg_kernel_times(dnsg_t *h, int n, double *ms);
+
+/* Instrumentation (no counterpart in the reference): band substeps one band launch
+ * of dnsg_step covers -- 2 where the handle runs both gratio-2 substeps in one pass
+ * (the default at gratio 2), else 1. */
+int dnsg_band_launch_substeps(const dnsg_t *h, int *substeps);
 
 /* Instrumentation (no counterpart in the reference): achievable HBM bandwidth of
  * a plain kernel over a buffer of `bytes` -- kind 0 copy (read + write),
@@ -260,7 +267,7 @@ int dnsg_hbm_probe(size_t bytes, int kind, int reps, double *gbs);
 
 /* encalc2.c:17-147 -> V.force[0] (flux mode) and the flowrate/density scalars. */
 int dnsg_encalc2(dnsg_t *h, long ss, dnsg_diag *d);
-/* wshearstr.c:15-169 -> str-up/str-do/w-str/p-grad scalars; DNSG_ERR_NAN on NaN. */
+/* wshearstr.c:15-168 -> str-up/str-do/w-str/p-grad scalars; DNSG_ERR_NAN on NaN. */
 int dnsg_wshearstr(dnsg_t *h, long ss, dnsg_diag *d);
 /* xcomputations.c:16-950: coarse collide-stream, time-level rotation, sampling. */
 int dnsg_xcomputations(dnsg_t *h);
@@ -325,6 +332,21 @@ int dnsg_spectra(dnsg_t *h, int grid, double *ex, double *ey);
  * Needs ydv >= 4 and zdv >= 6. */
 int dnsg_budget2d(dnsg_t *h, double *turb, double *vort, double *ke, double *ufl, double *vfl);
 
+/* Time averages of dnsg_budget2d's profiles over snapshots, in the accumulator
+ * convention of the reference's serial post-processor (TKE-budget/postprocessing,
+ * which averages the per-snapshot files: time_stat.c:132-210, vorticity.c:73-98,
+ * Ebudget.c:184-300): the mean of each profile over the snapshots, except the rms
+ * profiles -- turb rows UTIN VTIN WTIN DTIN and the three vort rows -- whose squares
+ * are averaged (the tool takes the square root when it writes them).
+ * dnsg_budget2d_accumulate runs dnsg_budget2d on the current state, folds its
+ * profiles into the handle's FP64 sums and, where turb / vort / ke are not NULL,
+ * returns them as dnsg_budget2d does (ufl / vfl are not formed).
+ * dnsg_budget2d_average returns the averages and *count snapshots (DNSG_ERR_STATE
+ * before the first fold); dnsg_budget2d_reset clears the sums. */
+int dnsg_budget2d_accumulate(dnsg_t *h, double *turb, double *vort, double *ke);
+int dnsg_budget2d_average(dnsg_t *h, double *turb, double *vort, double *ke, long *count);
+int dnsg_budget2d_reset(dnsg_t *h);
+
 /* ---- serving queue: steady slip-length solves, continuously batched -----------
  * No counterpart in the reference, which runs one channel per MPI job.  It is
  * for parametric sweeps: many independent channels of uneven sizes, each driven
@@ -334,7 +356,7 @@ int dnsg_budget2d(dnsg_t *h, double *turb, double *vort, double *ke, double *ufl
  * Requests wait in arrival order and are admitted, first fit, while the
  * resident solves' device bytes stay within the budget and fewer than
  * max_active are resident.  All resident solves of one precision advance
- * together: every kernel of the step runs as one launch over up to 16 solves
+ * together: every kernel of the step runs as one launch over up to 30 solves
  * (collide-stream, interface sampling, interface fill per gratio, injection,
  * flux sums).  Every check_every steps the bulk velocities of all resident
  * solves are read back in one copy; a solve whose bulk velocity changed by at
@@ -376,7 +398,11 @@ int dnsg_queue_destroy(dnsg_queue_t *q);
 const char *dnsg_queue_last_error(const dnsg_queue_t *q);
 /* seconds since the queue was created */
 double dnsg_queue_clock(const dnsg_queue_t *q);
-/* device bytes one solve holds while resident (0 if the parameters are invalid) */
+/* device bytes one solve takes while resident, as admission estimates it before
+ * creating the solve: its buffers, each of 1 MiB or more rounded up to the runtime's
+ * 2 MiB granularity, plus 2 MiB for the small ones (0 if the parameters are invalid).
+ * Once admitted, the queue accounts the larger of this and the measured footprint
+ * (free device memory before and after creating the solve). */
 size_t dnsg_solve_bytes(const dnsg_params *p);
 /* queue one solve; t_arrival < 0 stamps the current queue clock.  Rejects invalid
  * parameters, force_mode other than DNSG_FORCE_CONST, and a solve larger than
@@ -388,6 +414,12 @@ int dnsg_queue_submit(dnsg_queue_t *q, const dnsg_params *p, uint64_t id, double
  * Finished solves beyond cap are kept for the next call. */
 int dnsg_queue_advance(dnsg_queue_t *q, int steps, dnsg_slip_result *done, int cap, int *ndone);
 int dnsg_queue_info(const dnsg_queue_t *q, int *pending, int *active, size_t *resident_bytes);
+/* Check intervals run as a replayed HIP graph so far.  An interval (check_every
+ * coarse steps of every resident solve) is captured once the resident set has stayed
+ * the same for a whole interval and replayed while it stays so; the results are
+ * those of the launches it records.  DNSG_QUEUE_GRAPH=0 in the environment at
+ * dnsg_queue_create runs every interval with direct launches. */
+int dnsg_queue_graph_launches(const dnsg_queue_t *q, long *launches);
 
 #ifdef __cplusplus
 }

@@ -1080,10 +1080,10 @@ void orc_encalc2(orc_t *o, long ss, dnsg_diag *dg) {
     }
 }
 
-/* ---- wall shear stress: wshearstr.c:15-169 -------------------------------------- */
+/* ---- wall shear stress: wshearstr.c:15-168 -------------------------------------- */
 int orc_wshearstr(orc_t *o, long ss, dnsg_diag *dg) {
     const int zdf = o->zdf, NZs = o->NZs, GR = o->GR;
-    double mu = (1. / 6.) * (2. / o->tauf - 1); /* :181 */
+    double mu = (1. / 6.) * (2. / o->tauf - 1); /* :35 */
     double stdown = 0., stup = 0., davg = 0., pgrad, stress;
     const double area = ((double)((o->Xf + 1 - 1.) * (o->Yf + 1 - 1.) * 1 * 1));
     int i, j;
@@ -1096,7 +1096,7 @@ int orc_wshearstr(orc_t *o, long ss, dnsg_diag *dg) {
         (out) = x_ / d_;                                          \
         (densout) = d_;                                           \
     } while (0)
-    for (i = 0; i < o->Xf; i++)            /* :183-216 */
+    for (i = 0; i < o->Xf; i++)            /* :37-70 */
         for (j = 0; j < o->Yf; j++) {
             double ux1, ux2, ux3, dens;
             UX_AT(o->lft, 1, ux1, dens);
@@ -1105,11 +1105,11 @@ int orc_wshearstr(orc_t *o, long ss, dnsg_diag *dg) {
             UX_AT(o->lft, 3, ux3, dens);
             stdown += (-2. * ux1 + 3. * ux2 - ux3);
         }
-    davg /= area;   /* :218 */
-    stdown /= area; /* :219 */
-    stdown *= mu * davg; /* :227 */
+    davg /= area;   /* :72 */
+    stdown /= area; /* :73 */
+    stdown *= mu * davg; /* :81 */
     davg = 0.;
-    for (i = 0; i < o->Xf; i++)            /* :231-264 */
+    for (i = 0; i < o->Xf; i++)            /* :85-118 */
         for (j = 0; j < o->Yf; j++) {
             double ux1, ux2, ux3, dens;
             UX_AT(o->uft, zdf - 2, ux1, dens);
@@ -1121,10 +1121,10 @@ int orc_wshearstr(orc_t *o, long ss, dnsg_diag *dg) {
 #undef UX_AT
     davg /= area;
     stup /= area;
-    pgrad = o->force[0];                    /* :276-278 */
+    pgrad = o->force[0];                    /* :130-132 */
     pgrad = pgrad * (NZs - 2.) / (2. * GR);
-    stup *= mu * davg;                      /* :280 */
-    stress = 0.5 * (stup + stdown);         /* :284 */
+    stup *= mu * davg;                      /* :134 */
+    stress = 0.5 * (stup + stdown);         /* :138 */
     if (dg) {
         dg->time = ((double)o->p.TSTR + o->d.dt * ss);
         dg->str_up = stup;
@@ -1442,7 +1442,7 @@ int orc_append_logs(const char *dir, const dnsg_diag *d) {
     int rc = 0;
     rc |= append_line(dir, "flowrate.txt", d->time, d->flowrate); /* encalc2.c:137-139 */
     rc |= append_line(dir, "density.txt", d->time, d->density);   /* :141-143 */
-    rc |= append_line(dir, "str-up.txt", d->time, d->str_up);     /* wshearstr.c:293-296 */
+    rc |= append_line(dir, "str-up.txt", d->time, d->str_up);     /* wshearstr.c:146-149 */
     rc |= append_line(dir, "str-do.txt", d->time, d->str_down);   /* :298-301 */
     rc |= append_line(dir, "w-str.txt", d->time, d->stress);      /* :303-306 */
     rc |= append_line(dir, "p-grad.txt", d->time, d->pgrad);      /* :308-311 */

@@ -105,7 +105,7 @@ void orc_collide_node(double *f, double G, double omega);
 /* reference file layouts */
 int orc_write_checkpoint(const orc_t *o, const char *dir, long index); /* main:1262-1341 */
 int orc_read_checkpoint(orc_t *o, const char *dir, long index);        /* main:999-1183 */
-/* append the per-step text logs (encalc2.c:135-144, wshearstr.c:291-312) */
+/* append the per-step text logs (encalc2.c:135-144, wshearstr.c:145-166) */
 int orc_append_logs(const char *dir, const dnsg_diag *d);
 /* turb-fld.000.NNN (statistics.c:303-316) */
 int orc_write_turbfld(const char *dir, long ts, const double *stats, int NZ);

@@ -20,7 +20,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import ORC_CLI
+from conftest import ORC_CLI, REPO
 from helpers import PKG_DIR
 
 DRIVER = os.path.join(PKG_DIR, "build", "dnsg-run")
@@ -262,6 +262,62 @@ def test_driver_writes_budget(tmp_path, oracle_built):
         a, b = a.reshape(shape), b.reshape(shape)
         scale = budget_scales(kind, b) + 1e-300
         assert np.all(np.max(np.abs(a - b), axis=1) <= 1e-10 * scale), name
+
+
+POSTPROC = os.path.join(REPO, "oracle", "_ref", "postproc_micro", "postprocess-budget")
+
+
+def _dat_numbers(path):
+    out = []
+    for line in open(path):
+        try:
+            out.extend(float(x) for x in line.split())
+        except ValueError:
+            continue  # the Tecplot title / variables / zone lines
+    return np.array(out)
+
+
+@pytest.mark.gpu
+def test_reference_postprocessor_reads_driver_files(tmp_path, oracle_built):
+    """The reference's own TKE-budget post-processor (TKE-budget/postprocessing, built
+    from its sources by oracle/build_postproc_ref.sh; it averages the per-snapshot
+    budget files over time, time_stat.c / vorticity.c / Ebudget.c, and writes them in
+    wall units with u_tau from p-grad.txt and U_b from flowrate.txt) run unchanged in
+    the driver's run directory, and in a directory holding the oracle's files of the
+    same run: it reads both and writes the same 81 profile files, every number within
+    1e-8 + 1e-6 relative (the per-snapshot inputs agree to 1e-10 of each profile's
+    scale; the tool prints 9 decimals).  The density skewness and kurtosis are left
+    out (skew-p*, kurt-p* and the last column of skew-all / kurt-all): the seeded start
+    has density fluctuations at rounding level, so those moments are ratios of rounding
+    noise and differ between any two correct implementations."""
+    if not os.path.exists(POSTPROC):
+        pytest.skip("reference post-processor not built (oracle/build_postproc_ref.sh needs the reference sources)")
+    kw = dict(MICRO, **SEEDED, force_mode=1, g_const="2e-5", ITER=3)
+    odir, gdir, _ = _run_pair(tmp_path, kw, "p")
+    r = subprocess.run([DRIVER, "-p"] + _args(kw) + ["results_dat=0", "budget=1"], cwd=gdir, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    post = odir / "post"
+    post.mkdir()
+    subprocess.run([ORC_CLI, "budget", str(post)] + _args(kw) + ["TSTR=0", "TEND=3"], cwd=odir, check=True)
+    for f in ("p-grad.txt", "flowrate.txt"):
+        shutil.copy(odir / f, post / f)
+    for d in (gdir, post):
+        rp = subprocess.run([POSTPROC], cwd=d, capture_output=True, text=True, timeout=120)
+        assert rp.returncode == 0, (str(d), rp.stdout[-1000:], rp.stderr[-1000:])
+    names = sorted(n for n in os.listdir(post) if n.endswith(".dat"))
+    assert len(names) == 81, len(names)
+    for name in names:
+        if name.startswith(("skew-p", "kurt-p")):
+            continue
+        a, b = _dat_numbers(gdir / name), _dat_numbers(post / name)
+        assert a.shape == b.shape and a.size > 0, name
+        if name in ("skew-all.dat", "kurt-all.dat"):  # rows: k, z+, u, w, v, p
+            keep = np.arange(a.size) % 6 != 5
+            a, b = a[keep], b[keep]
+        assert np.array_equal(np.isfinite(a), np.isfinite(b)), name
+        f = np.isfinite(b)
+        assert np.all(np.abs(a[f] - b[f]) <= 1e-8 + 1e-6 * np.abs(b[f])), (name, np.max(np.abs(a[f] - b[f])))
 
 
 def test_driver_rejects_unknown_key(tmp_path):
