@@ -57,6 +57,7 @@ class Dims(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+NOALLOC_MAX_STEPS = 4096  # DNSG_NOALLOC_MAX_STEPS of include/dnsg.h
 DIAG_FIELDS = ["time", "force", "momentum", "density", "flowrate", "str_up", "str_down", "stress", "pgrad"]
 
 
@@ -101,6 +102,7 @@ SIGNATURES = {
     "dnsg_upload_state_ghosted": [_P, _DP, _DP, _DP, ctypes.POINTER(_DP), ctypes.POINTER(_DP), ctypes.c_double],
     "dnsg_download_state_ghosted": [_P, _DP, _DP, _DP, ctypes.POINTER(_DP), ctypes.POINTER(_DP), _DP],
     "dnsg_step": [_P, ctypes.c_int, ctypes.POINTER(Diag)],
+    "dnsg_step_noalloc": [_P, ctypes.c_int, ctypes.POINTER(Diag)],
     "dnsg_run_timed": [_P, ctypes.c_int, ctypes.POINTER(Diag), _DP],
     "dnsg_run_timed_sections": [_P, ctypes.c_int, ctypes.POINTER(Diag), _DP, _DP, ctypes.c_int],
     "dnsg_kernel_times": [_P, ctypes.c_int, _DP],
@@ -281,6 +283,13 @@ class Solver:
     def step(self, n=1):
         d = (Diag * max(n, 1))()
         self._check(self.lib.dnsg_step(self.h, n, d))
+        return diag_array(d, n)
+
+    def step_noalloc(self, n=1):
+        """dnsg_step_noalloc: the same steps with no device or host allocation in the
+        library (records through its fixed device array; n <= NOALLOC_MAX_STEPS)"""
+        d = (Diag * max(n, 1))()
+        self._check(self.lib.dnsg_step_noalloc(self.h, n, d))
         return diag_array(d, n)
 
     def budget2d_accumulate(self):

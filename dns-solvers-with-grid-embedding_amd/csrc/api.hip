@@ -10,6 +10,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -140,6 +141,43 @@ int ensure_pinned(dnsg_t *h, size_t n) {
 }
 
 }  // namespace
+
+// ---- scratch of the allocation-free step ------------------------------------------
+// dnsg_step_noalloc keeps the NaN halt flag (first 256 bytes) and the records of up to
+// DNSG_NOALLOC_MAX_STEPS queued steps in this file-scope device array.  Each GPU holds
+// its own copy until the process ends.  Host code never passes the array's name to a
+// kernel or a copy (on the host that name is a stand-in, not a device address): the
+// address on each device comes from hipGetSymbolAddress, taken once per device by
+// dnsg_create with that device current, and kept in scratch_addr.
+namespace {
+constexpr size_t NOALLOC_REC_OFS = 32;  // doubles before the records: the halt flag, padded to 256 B
+constexpr size_t NOALLOC_WORDS = NOALLOC_REC_OFS + (size_t)DNSG_NOALLOC_MAX_STEPS * DIAG_WORDS;
+constexpr int NOALLOC_DEVICES = 64;
+}  // namespace
+static __device__ __align__(256) double scratch[NOALLOC_WORDS];
+static double *scratch_addr[NOALLOC_DEVICES];
+static hipError_t scratch_status[NOALLOC_DEVICES];
+static std::once_flag scratch_once[NOALLOC_DEVICES];
+
+// the device address of this handle's device's scratch (h->device is current when
+// dnsg_create calls it); base may be null
+static int noalloc_scratch(dnsg_t *h, double **base) {
+    if (h->device < 0 || h->device >= NOALLOC_DEVICES) {
+        h->err = "device index beyond the scratch table";
+        return DNSG_ERR_ARG;
+    }
+    std::call_once(scratch_once[h->device], [h]() {
+        void *p = nullptr;
+        scratch_status[h->device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        scratch_addr[h->device] = static_cast<double *>(p);
+    });
+    if (scratch_status[h->device] != hipSuccess || !scratch_addr[h->device]) {
+        h->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(scratch_status[h->device]);
+        return DNSG_ERR_HIP;
+    }
+    if (base) *base = scratch_addr[h->device];
+    return DNSG_OK;
+}
 
 namespace dnsg {
 
@@ -872,6 +910,8 @@ int dnsg::create_ctx(const dnsg_params *p, hipStream_t ext, dnsg_t **out) {
     HIPCHK(h, hipMalloc(&h->d_halt, sizeof(int)));
     HIPCHK(h, hipMemsetAsync(h->d_halt, 0, sizeof(int), h->stream));
     h->halt_flag = h->d_halt;
+    if ((rc = noalloc_scratch(h, nullptr))) return rc;
+    h->err.reserve(256);  // error texts of dnsg_step_noalloc fit without a reallocation
     const size_t nred = 4 * ((size_t)d.zdc + 2 * (d.zdf - 1)) + 2 * 9 * (size_t)d.NZ + 64;
     HIPCHK(h, hipMalloc(&h->d_red, nred * 8));
     HIPCHK(h, hipMemsetAsync(h->d_red, 0, nred * 8, h->stream));
@@ -1136,6 +1176,25 @@ int dnsg_step(dnsg_t *h, int n, dnsg_diag *diag) {
         if (int rc = run_steps(h, m, diag ? diag + s0 : nullptr, nullptr, nullptr)) return rc;
     }
     return DNSG_OK;
+}
+
+int dnsg_step_noalloc(dnsg_t *h, int n, dnsg_diag *diag) {
+    if (!h || !h->stream || n < 0) return DNSG_ERR_ARG;
+    if (n > DNSG_NOALLOC_MAX_STEPS) {
+        h->err = "dnsg_step_noalloc: n exceeds DNSG_NOALLOC_MAX_STEPS (the records of the queued steps "
+                 "live in a fixed device array)";
+        return DNSG_ERR_ARG;
+    }
+    if (!h->have_state) {
+        h->err = "no state: call dnsg_init_state or dnsg_upload_state first";
+        return DNSG_ERR_STATE;
+    }
+    if (n == 0) return DNSG_OK;
+    double *base = nullptr;
+    int rc;
+    if ((rc = noalloc_scratch(h, &base))) return rc;
+    const StepBufs B = {base + NOALLOC_REC_OFS, reinterpret_cast<int *>(base), false};
+    return run_steps(h, n, diag, nullptr, nullptr, &B);
 }
 
 int dnsg_run_timed(dnsg_t *h, int n, dnsg_diag *diag, double *elapsed_ms) {

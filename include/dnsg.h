@@ -225,6 +225,22 @@ int dnsg_download_state(dnsg_t *h, double *coarse, double *lower, double *upper,
  * stress is NaN. */
 int dnsg_step(dnsg_t *h, int n, dnsg_diag *diag);
 
+/* No counterpart in the reference: dnsg_step for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * dnsg_step may grow the handle's device record buffer and its pinned host staging
+ * when n exceeds what an earlier call used; this variant allocates no device or host
+ * memory on its success path.  The records of the n queued steps and the NaN halt
+ * flag live in a fixed device array of the library (DNSG_NOALLOC_MAX_STEPS records,
+ * about 288 KB, held by each GPU the library runs on until the process ends), and
+ * come back through a buffer on the host stack.  All n steps are queued before the
+ * one synchronisation, and the results, the records and the NaN behaviour are those
+ * of dnsg_step.  n > DNSG_NOALLOC_MAX_STEPS returns DNSG_ERR_ARG.  Every handle on
+ * one GPU shares that array, which keeps the last call's contents: calls that
+ * overlap in time on one GPU (two host threads, two handles) race and must be
+ * serialised by the caller. */
+#define DNSG_NOALLOC_MAX_STEPS 4096
+int dnsg_step_noalloc(dnsg_t *h, int n, dnsg_diag *diag);
+
 /* Same as dnsg_step but without per-step host synchronisation; the diag records
  * are copied back once at the end.  *elapsed_ms receives the HIP-event time of
  * the n steps (device time, inputs resident in HBM). */
