@@ -252,19 +252,25 @@ def precision_tier(mod, steps=500):
     return res
 
 
-QUEUE_ENVELOPE = dict(x_range=(32, 128), y_range=(32, 128), z_range=(16, 32), re_range=(10.0, 50.0))
+# The serving workload's envelope (the slip_workload defaults): streamwise and spanwise
+# 32..256, wall-normal 16..64 coarse cells, Re_b 10..200.  "small" is a reduced envelope
+# (wall-normal 16..32, Re_b 10..50) whose solves converge in seconds.
+QUEUE_ENVELOPES = {"full": dict(x_range=(32, 256), y_range=(32, 256), z_range=(16, 64), re_range=(10.0, 200.0)),
+                   "small": dict(x_range=(32, 128), y_range=(32, 128), z_range=(16, 32), re_range=(10.0, 50.0))}
+QUEUE_STREAM = 256  # requests in the seeded stream; a run takes its first n
 
 
-def run_queue(mod, n, rate, seed, n_check):
-    """Slip-length serving queue: n requests from the seeded generator (streamwise and
-    spanwise 32..128, wall-normal 16..32 coarse cells, bands on a random half, stripes
-    or posts, gas fraction 0.2..0.9, period 8..64 fine cells, Re_b 10..50, shifted FP32),
-    Poisson arrivals at `rate` per second, each solved until its bulk velocity moves by
-    < 1e-9 (relative) over 1000 steps.  Latency = completion - arrival.  The first
-    n_check requests are solved again in FP64 (plain path) and their slip lengths
-    compared."""
+def run_queue(mod, n, rate, seed, n_check, envelope="full", max_steps=400000):
+    """Slip-length serving queue: the first n requests of the seeded 256-request stream
+    (mod.slip_workload over the envelope: bands on a random half, stripes or posts, gas
+    fraction 0.2..0.9, period 8..64 fine cells, shifted FP32), Poisson arrivals at `rate`
+    per second, each solved until its bulk velocity moves by < 1e-9 (relative) over
+    1000 steps (or, at the FP32 storage floor, stays within 3e-5 over 5 windows).
+    Latency = completion - arrival.  The first n_check requests are solved again in
+    FP64 (plain path, the 1e-9 criterion) and their slip lengths compared."""
     import random
-    work = mod.slip_workload(n, seed, precision="fp32", **QUEUE_ENVELOPE)
+    env = QUEUE_ENVELOPES[envelope]
+    work = mod.slip_workload(QUEUE_STREAM, seed, precision="fp32", **env)[:n]
     rng = random.Random(seed + 1)
     t, arrivals = 0.0, []
     for _ in range(n):
@@ -274,7 +280,7 @@ def run_queue(mod, n, rate, seed, n_check):
     for d, p in work:
         dm = mod.dims_of(p)
         lu[d["id"]] = dm.Xc * dm.Yc * dm.zdc + dm.GR * 2 * dm.Xf * dm.Yf * (dm.zdf - 1)
-    q = mod.Queue(max_active=n, check_every=100, window=1000, tol=1e-9, max_steps=400000)
+    q = mod.Queue(max_active=n, check_every=100, window=1000, tol=1e-9, max_steps=max_steps)
     t0 = q.clock()
     done, i, last = {}, 0, time.time()
     while len(done) < n:
@@ -293,12 +299,14 @@ def run_queue(mod, n, rate, seed, n_check):
                   flush=True)
             last = time.time()
     wall = q.clock() - t0
+    graphs = q.graph_launches()
     q.close()
     lat = np.array([done[k]["t_done"] - done[k]["t_arrival"] for k in range(n)])
     steps = np.array([done[k]["steps"] for k in range(n)])
     glu = sum(done[k]["steps"] * lu[k] for k in range(n)) / 1e9
-    out = dict(requests=n, arrival_rate_per_s=rate, seed=seed, precision="fp32 shifted, fp64 reductions",
-               envelope=QUEUE_ENVELOPE, converged_tol=int(sum(done[k]["status"] == 0 for k in range(n))),
+    out = dict(requests=n, stream=f"first {n} of the {QUEUE_STREAM}-request stream of seed {seed}",
+               arrival_rate_per_s=rate, seed=seed, precision="fp32 shifted, fp64 reductions",
+               envelope=dict(name=envelope, **env), converged_tol=int(sum(done[k]["status"] == 0 for k in range(n))),
                steady_at_fp32_floor=int(sum(done[k]["status"] == 2 for k in range(n))),
                capped=int(sum(done[k]["status"] == 1 for k in range(n))),
                criterion="|dU/U| <= 1e-9 over 1000 steps, or (FP32) within 3e-5 of U 1..5 windows back",
@@ -306,8 +314,9 @@ def run_queue(mod, n, rate, seed, n_check):
                p99_s=float(np.percentile(lat, 99)), mean_s=float(lat.mean()), wall_s=wall,
                throughput_req_per_s=n / wall, glups=glu / wall, steps_median=int(np.median(steps)),
                steps_max=int(steps.max()))
+    out["graph_intervals"] = graphs
     if n_check:
-        q = mod.Queue(max_active=n_check, check_every=100, window=1000, tol=1e-9, max_steps=400000)
+        q = mod.Queue(max_active=n_check, check_every=100, window=1000, tol=1e-9, max_steps=max_steps)
         for k in range(n_check):
             d = work[k][0]
             q.submit(mod.slip_request(d["xdv"], d["ydv"], d["zdv"], d["gratio"], d["pattern"], d["period"],
@@ -320,7 +329,14 @@ def run_queue(mod, n, rate, seed, n_check):
         db = [abs(done[k]["slip_length"] - ref[k]["slip_length"]) for k in range(n_check)]
         out["fp64_check"] = dict(requests=n_check, max_abs_diff_slip=float(max(db)),
                                  max_rel_diff_slip=float(max(db[k] / max(abs(ref[k]["slip_length"]), 1e-12)
-                                                             for k in range(n_check))))
+                                                             for k in range(n_check))),
+                                 fp64_converged_tol=int(sum(ref[k]["status"] == 0 for k in range(n_check))),
+                                 per_request=[dict(id=k, slip_fp32=done[k]["slip_length"],
+                                                   slip_fp64=ref[k]["slip_length"], status_fp32=done[k]["status"],
+                                                   status_fp64=ref[k]["status"], steps_fp32=done[k]["steps"],
+                                                   steps_fp64=ref[k]["steps"],
+                                                   latency_s=done[k]["t_done"] - done[k]["t_arrival"])
+                                              for k in range(n_check)])
     return out
 
 
@@ -332,9 +348,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tiers", default="fp64,fp32,fp16")
-    ap.add_argument("--queue-requests", type=int, default=32)
+    ap.add_argument("--queue-requests", type=int, default=4,
+                    help="the first n requests of the seeded 256-request stream")
     ap.add_argument("--queue-rate", type=float, default=1.0, help="Poisson arrival rate, requests per second")
-    ap.add_argument("--queue-check", type=int, default=8, help="requests re-solved in FP64 for the slip check")
+    ap.add_argument("--queue-check", type=int, default=4, help="requests re-solved in FP64 for the slip check")
+    ap.add_argument("--queue-envelope", choices=sorted(QUEUE_ENVELOPES), default="full")
+    ap.add_argument("--queue-only", action="store_true", help="run the serving queue only and print its record")
     ap.add_argument("--no-queue", action="store_true")
     ap.add_argument("--no-precision-report", action="store_true")
     ap.add_argument("--no-cfg5", action="store_true")
@@ -355,6 +374,12 @@ def main():
         s.step(3)
         s.close()
         print(f"bench: copy probe {mod.hbm_probe(1 << 30, 0, 1):.1f} GB/s", file=sys.stderr, flush=True)
+        return
+
+    if args.queue_only:  # the serving workload alone: its record as one JSON line
+        queue = run_queue(mod, args.queue_requests, args.queue_rate, 2025, min(args.queue_check, args.queue_requests),
+                          args.queue_envelope)
+        print(json.dumps({"slip_queue": queue}), flush=True)
         return
 
     tiers = [t for t in args.tiers.split(",") if t]
@@ -396,7 +421,8 @@ def main():
     queue = None
     if not args.no_queue and args.queue_requests > 0:
         print(f"bench: slip queue ({args.queue_requests} requests at {args.queue_rate}/s)", file=sys.stderr, flush=True)
-        queue = run_queue(mod, args.queue_requests, args.queue_rate, 2025, min(args.queue_check, args.queue_requests))
+        queue = run_queue(mod, args.queue_requests, args.queue_rate, 2025, min(args.queue_check, args.queue_requests),
+                          args.queue_envelope)
         print(f"bench: queue p50 {queue['p50_s']:.2f} s p99 {queue['p99_s']:.2f} s, {queue['glups']:.2f} GLUPS",
               file=sys.stderr, flush=True)
     cpu = None

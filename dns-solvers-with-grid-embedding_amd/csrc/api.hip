@@ -865,10 +865,13 @@ int dnsg::create_ctx(const dnsg_params *p, hipStream_t ext, dnsg_t **out) {
     }
     h->elem = p->precision == DNSG_PREC_FP64 ? 8 : (p->precision == DNSG_PREC_FP32 ? 4 : 2);
     {
-        // gratio 2: both band substeps in one pass (k_band2.hip); DNSG_BAND2=0 in the
-        // environment selects the one-launch-per-substep path instead (same results)
+        // gratio 2: both band substeps in one pass (k_band2.hip), by default in the FP64
+        // constant-force configuration, where it is the faster path; DNSG_BAND2=1 /
+        // DNSG_BAND2=0 in the environment selects it / the one-launch-per-substep path
+        // for any configuration (same results)
         const char *e = getenv("DNSG_BAND2");
-        h->band2 = h->d.GR == 2 && (e && e[0] == '1');
+        const bool pick = p->precision == DNSG_PREC_FP64 && p->force_mode == DNSG_FORCE_CONST;
+        h->band2 = h->d.GR == 2 && ((e && e[0]) ? e[0] == '1' : pick);
     }
     const dnsg_dims &d = h->d;
     h->coarse.nx = d.Xc; h->coarse.ny = d.Yc; h->coarse.nz = d.zdc;

@@ -80,6 +80,15 @@ struct Lds {
     T hf[2][NR][2];
     T hu[3][NR][2];
     T hd[NR][2];
+    int8_t wcl[NR][CW];  // the tile's wall codes, ring included
+};
+
+// wall codes of scatter_push from the tile's LDS copy: the node at row wave r, LDS
+// column col, and its in-plane neighbours
+struct WallCodesTile {
+    const int8_t (*w)[CW];
+    int r, col;
+    __device__ __forceinline__ int operator()(int, int dx, int dy, int, int) const { return w[r + dy][col + dx]; }
 };
 
 template <class S>
@@ -97,7 +106,8 @@ __device__ __forceinline__ void band2_tile(const LatArgs &L, int tile, Lds<typen
     const int tx = tile % tilesx, ty = tile / tilesx;
     const int x0 = tx * TX, y0 = ty * TY;
     const int w = min(TX, nx - x0), th = min(TY, ny - y0), nrows = th + 2;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wave index as a scalar, so that the compiler sees branches on it as uniform
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int cpr = tilesx;  // partials per (layer, row)
     const S *__restrict__ cur = static_cast<const S *>(L.cur);
     S *__restrict__ nxt = static_cast<S *>(L.nxt);
@@ -120,11 +130,10 @@ __device__ __forceinline__ void band2_tile(const LatArgs &L, int tile, Lds<typen
     T uprv2[3] = {0, 0, 0}, uprv[3] = {0, 0, 0}, ucur[3] = {0, 0, 0};
     T dcur[3] = {0, 0, 0};
 
-    // gather, collide and push the second substep at layer t (core row waves only)
-    auto second = [&](int t, bool wall) __attribute__((always_inline)) {
+    // gather and collide the second substep at layer t into g (core row waves only)
+    auto second = [&](int t, bool wall, T *g) __attribute__((always_inline)) {
         const int m = SIDE ? t : nz - 1 - t;
         const size_t row = (size_t)m * ny + y;
-        T g[Q];
         if (t == 0) {  // the interface layer: the first fill's values
             unroll<0, Q>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
@@ -166,13 +175,12 @@ __device__ __forceinline__ void band2_tile(const LatArgs &L, int tile, Lds<typen
             if (wall) {
                 // directions leaving the wall (toward the interface) at the layer next to
                 // it, from that layer's toward-the-wall values (layer t, slot sw, uprv)
-                const int8_t own = act ? wc[(size_t)y * nx + x] : (int8_t)0;
+                const int own = act ? (int)sm.wcl[r][col] : 0;
                 unroll<0, Q>([&](auto qc) {
                     constexpr int b = decltype(qc)::value;
                     if constexpr (cls<SIDE>(b) == 2) {
                         constexpr int ebx = EX[b], eby = EY[b];
-                        const int xs = (x - ebx + nx) % nx, ys = (y - eby + ny) % ny;
-                        const bool bb = act && own == -1 && wc[(size_t)ys * nx + xs] == -1;
+                        const bool bb = act && own == -1 && sm.wcl[r - eby][col - ebx] == -1;
                         T vbb, vsp;
                         if constexpr (eby == 0) {
                             vbb = uprv[xi(OPP[b])];
@@ -196,8 +204,13 @@ __device__ __forceinline__ void band2_tile(const LatArgs &L, int tile, Lds<typen
             }
         }
         collide_any(g, G, om);
+    };
+    // push the second substep's result at layer t to its targets
+    auto push = [&](int t, const T *g) __attribute__((always_inline)) {
+        const int m = SIDE ? t : nz - 1 - t;
+        const size_t row = (size_t)m * ny + y;
         RowSums rs;
-        if (act) scatter_push<S, DS>(L, nxt, (int)row, x, g, rs);
+        if (act) scatter_push<S, DS>(L, nxt, (int)row, x, g, rs, WallCodesTile{sm.wcl, r, col});
         if (DS && L.dsum) {
             const double dm = wave_sum64(rs.dm), dd = wave_sum64(rs.dd);
             if (lane == 0) {
@@ -208,16 +221,36 @@ __device__ __forceinline__ void band2_tile(const LatArgs &L, int tile, Lds<typen
         }
     };
 
+    // the first substep's input of a layer, loaded one layer ahead: the loads of layer
+    // t+1 are issued at the top of iteration t and waited for just before the second
+    // substep's stores (below), so they overlap both collisions.  The loads are
+    // unconditional, so no join between lanes or waves waits for them: lanes without
+    // a node read the tile's first ring node, whose cache lines lane 0 of the first
+    // row wave fetches anyway, and take rest values at use.
+    const int yl = act ? y : ((y0 - 1) % ny + ny) % ny, xl = act ? x : x0;
+    T fin[Q];
+    auto load_layer = [&](int t) __attribute__((always_inline)) {
+        const size_t row = (size_t)(SIDE ? t : nz - 1 - t) * ny + yl;
+        unroll<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            fin[q] = (T)load_s(cur + lat_at(q, row, xl, nx));
+        });
+    };
+    // the tile's wall codes, read at the layer next to the wall
+    if (act) sm.wcl[r][col] = wc[(size_t)y * nx + x];
+    load_layer(0);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // the first layer is waited for on entry
     for (int t = 0; t < nz; t++) {
         // ---- first substep at layer t: load, sum, collide, share
         const int m = SIDE ? t : nz - 1 - t;
         const size_t row = (size_t)m * ny + y;
+        T f[Q];
+        unroll<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            f[q] = act ? fin[q] : rest_value<S>(q);
+        });
+        if (t + 1 < nz) load_layer(t + 1);
         if (rowok) {
-            T f[Q];
-            unroll<0, Q>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                f[q] = act ? (T)load_s(cur + lat_at(q, row, x, nx)) : rest_value<S>(q);
-            });
             if (core && L.red) {
                 RowSums rs;
                 if (act) rs.add<S>(f);
@@ -263,11 +296,18 @@ __device__ __forceinline__ void band2_tile(const LatArgs &L, int tile, Lds<typen
             });
         }
         __syncthreads();
-        // ---- second substep at layer t-1
-        if (t >= 1) {
-            if (core) second(t - 1, false);
-            __syncthreads();
-        }
+        // ---- second substep at layer t-1: gather and collide, then (after the barrier
+        // that frees the layer slots for the next iteration) push
+        T g[Q];
+        if (t >= 1 && core) second(t - 1, false, g);
+        // wait here, on every wave, for every vector memory operation in flight -- the
+        // next layer's loads and the previous layer's stores -- so that the pushes below
+        // are the only ones outstanding when the next iteration first uses the loaded
+        // values (the compiler's own wait there would cover them too).  The wait sits
+        // beside the barrier, in code that every path runs.
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), no wait on expcnt / lgkmcnt
+        __syncthreads();
+        if (t >= 1 && core) push(t - 1, g);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             fprv[k] = fcur[k];
@@ -276,7 +316,11 @@ __device__ __forceinline__ void band2_tile(const LatArgs &L, int tile, Lds<typen
         }
     }
     // ---- second substep at the layer next to the wall
-    if (core) second(nz - 1, true);
+    if (core) {
+        T g[Q];
+        second(nz - 1, true, g);
+        push(nz - 1, g);
+    }
 }
 
 template <class S, bool DS>

@@ -65,7 +65,7 @@ __device__ __forceinline__ void cs_scatter(const LatArgs &L, const S *__restrict
     });
     if (red) rs.add<S>(f);
     collide_any(f, G, om);
-    scatter_push<S, DS>(L, nxt, row, x, f, rs);
+    scatter_push<S, DS>(L, nxt, row, x, f, rs, WallCodesGlobal{L});
 }
 
 // Aligned form for chunk x0 .. x0 + CS_THREADS - 1 of row (y, z) (nx a multiple of

@@ -78,9 +78,20 @@ __device__ __forceinline__ void add_write(RowSums &rs, double W, int ex, double 
 //               the source: s(t) = -1 and s1 = -1 -> halfway bounce-back
 //               f(x, opp q); otherwise specular reflection f(x + e_xy, mirror q)
 // With DS and L.dsum, the destination-weighted sums of the writes go into rs.
-template <class S, bool DS, class T>
+// The wall codes come from wcode(zs, dx, dy, xt, yt): the code of the wall at the
+// in-plane offset (dx, dy) from the node, (xt, yt) in lattice coordinates, on the lo
+// (zs < 0) or hi wall.
+struct WallCodesGlobal {
+    const LatArgs &L;
+    __device__ __forceinline__ int operator()(int zs, int, int, int xt, int yt) const {
+        const int8_t *wc = zs < 0 ? L.wlo : L.whi;
+        return wc[(size_t)yt * L.nx + xt];
+    }
+};
+
+template <class S, bool DS, class T, class WC>
 __device__ __forceinline__ void scatter_push(const LatArgs &L, S *__restrict__ nxt, int row, int x, const T *f,
-                                             RowSums &rs) {
+                                             RowSums &rs, const WC &wcode) {
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const int y = row % ny, z = row / ny;
     const bool lo_edge = (z == 0), hi_edge = (z == nz - 1);
@@ -97,9 +108,8 @@ __device__ __forceinline__ void scatter_push(const LatArgs &L, S *__restrict__ n
             if (at_end) {
                 const int kind = (EZ[q] < 0) ? L.lo_kind : L.hi_kind;
                 if (kind == BOUND_WALL) {
-                    const int8_t *wc = (EZ[q] < 0) ? L.wlo : L.whi;
-                    const int sld = wc[(size_t)yt * nx + xt];
-                    const int sld1 = wc[(size_t)y * nx + x];
+                    const int sld = wcode(EZ[q], EX[q], EY[q], xt, yt);
+                    const int sld1 = wcode(EZ[q], 0, 0, x, y);
                     if (sld == -1 && sld1 == -1) {
                         store_s(nxt + lat_at(OPP[q], (size_t)row, x, nx), f[q]);
                         if (ds) add_write(rs, wz[1], -EX[q], read_back<S, q>(f[q]));

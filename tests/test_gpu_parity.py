@@ -135,8 +135,16 @@ def test_upload_download_roundtrip():
         assert np.array_equal(g[k], ref[k]), k
 
 
+# band path: the handle's default (both gratio-2 substeps in one launch for FP64
+# constant force), one launch per substep, one launch for both
+BAND = ["default", "0", "1"]
+
+
+@pytest.mark.parametrize("band", BAND)
 @pytest.mark.parametrize("name", list(CASES))
-def test_constant_force_bitwise(name):
+def test_constant_force_bitwise(name, band, monkeypatch):
+    if band != "default":
+        monkeypatch.setenv("DNSG_BAND2", band)
     o, s = pair(CASES[name], force_mode="const", g_const=2e-5)
     n = 25
     dg = s.step(n)
@@ -167,8 +175,11 @@ def test_constant_force_bitwise_split_directions():
         assert np.array_equal(g[k], ref[k]), f"{k}: max diff {np.max(np.abs(g[k] - ref[k]))}"
 
 
+@pytest.mark.parametrize("band", BAND)
 @pytest.mark.parametrize("name", list(CASES))
-def test_flux_control_parity(name):
+def test_flux_control_parity(name, band, monkeypatch):
+    if band != "default":
+        monkeypatch.setenv("DNSG_BAND2", band)
     o, s = pair(CASES[name])
     n = 50
     dg = s.step(n)
