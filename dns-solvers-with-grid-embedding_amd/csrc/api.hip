@@ -195,6 +195,7 @@ LatArgs coarse_args(dnsg_t *h) {
     L.red = nullptr;
     L.force = h->d_force;
     L.dsum = nullptr;
+    L.paired = h->cs_pair && cs_paired(L.nx, h->elem);
     set_dest_weights(h, 0, &L);
     return L;
 }
@@ -219,6 +220,7 @@ void band_args(dnsg_t *h, LatArgs *L) {
         L[b].red = nullptr;
         L[b].force = h->d_force;
         L[b].dsum = nullptr;
+        L[b].paired = h->cs_pair && cs_paired(g.nx, h->elem);
         set_dest_weights(h, 1 + b, &L[b]);
     }
 }
@@ -869,6 +871,10 @@ int dnsg::create_ctx(const dnsg_params *p, hipStream_t ext, dnsg_t **out) {
         // constant-force configuration, where it is the faster path; DNSG_BAND2=1 /
         // DNSG_BAND2=0 in the environment selects it / the one-launch-per-substep path
         // for any configuration (same results)
+        // FP32 / FP16 rows of a multiple of 2 CS_THREADS nodes: the paired collide-stream
+        // form (k_stream.hip), unless DNSG_CS_PAIRED=0 (the scatter form; same state)
+        const char *cp = getenv("DNSG_CS_PAIRED");
+        h->cs_pair = !(cp && cp[0] == '0');
         const char *e = getenv("DNSG_BAND2");
         const bool pick = p->precision == DNSG_PREC_FP64 && p->force_mode == DNSG_FORCE_CONST;
         h->band2 = h->d.GR == 2 && ((e && e[0]) ? e[0] == '1' : pick);

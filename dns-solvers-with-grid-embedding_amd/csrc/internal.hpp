@@ -50,6 +50,7 @@ struct LatArgs {
     // klo..khi carry kscale (khalf: half of it), all others 0.
     double *dsum;
     int klo, khi, khalf;
+    int paired;  // the paired collide-stream form (cs_paired): two nodes per thread
     double kscale;
 };
 
@@ -86,8 +87,19 @@ constexpr int CS_THREADS = DNSG_CS_THREADS;
 __host__ __device__ __forceinline__ int cs_chunks(int nx) { return (nx + CS_THREADS - 1) / CS_THREADS; }
 // lattices narrower than a workgroup: each workgroup takes this many whole rows
 __host__ __device__ __forceinline__ int cs_rows_per_wg(int nx) { return nx >= CS_THREADS ? 1 : CS_THREADS / nx; }
+// Paired form of the shifted tiers (elem = 4 or 2 bytes per value; mask bit 1 FP32,
+// bit 2 FP16): rows that are a multiple of 2 CS_THREADS nodes are taken two chunks
+// per workgroup, two adjacent nodes per thread, with 2-value loads and stores.
+#ifndef DNSG_CS_PAIRED_MASK
+#define DNSG_CS_PAIRED_MASK 6
+#endif
+__host__ __device__ __forceinline__ bool cs_paired(int nx, int elem) {
+    const int bit = elem == 4 ? 2 : (elem == 2 ? 4 : 0);
+    return (DNSG_CS_PAIRED_MASK & bit) != 0 && nx % (2 * CS_THREADS) == 0;
+}
 // collide-stream workgroups of one lattice with `rows` rows of nx nodes
-__host__ __device__ __forceinline__ int cs_blocks(int nx, int rows) {
+__host__ __device__ __forceinline__ int cs_blocks(int nx, int rows, bool paired) {
+    if (paired) return rows * (cs_chunks(nx) / 2);
     const int rpw = cs_rows_per_wg(nx);
     return rpw == 1 ? rows * cs_chunks(nx) : (rows + rpw - 1) / rpw;
 }
@@ -137,6 +149,7 @@ struct dnsg_ctx {
     // fused-reduction partial in the bands' partials of the last launch that wrote them
     bool band2 = false;
     int band_cols = dnsg::CS_THREADS;
+    bool cs_pair = true;  // FP32 / FP16: the paired collide-stream form where the rows allow it
     int8_t *wall_lo = nullptr, *wall_up = nullptr;     // [Yf][Xf]
 
     // reductions

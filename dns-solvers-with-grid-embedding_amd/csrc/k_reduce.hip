@@ -26,9 +26,9 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// block-wide sum of K values; result valid in thread 0
-template <int K>
-__device__ __forceinline__ void block_sum(double (&v)[K], double (*lds)[RB / 64]) {
+// block-wide sum of K values over NB threads; result valid in thread 0
+template <int K, int NB = RB>
+__device__ __forceinline__ void block_sum(double (&v)[K], double (*lds)[NB / 64]) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -40,7 +40,7 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double (*lds)[RB / 64]
 #pragma unroll
         for (int k = 0; k < K; k++) {
             double s = 0.;
-            for (int w = 0; w < RB / 64; w++) s += lds[k][w];
+            for (int w = 0; w < NB / 64; w++) s += lds[k][w];
             v[k] = s;
         }
     }
@@ -447,20 +447,24 @@ __global__ __launch_bounds__(RB) void k_fin_wshear(WsArgs A) {
 
 // wall stress from the row sums of the first band substep (RED_W per row):
 // sum_x (-2u1 + 3u2 - u3) = -2 S1 + 3 S2 - S3 with S_m = sum_x u_x of layer m
-__global__ __launch_bounds__(RB) void k_fin_wshear_fused(WsArgs A, const double *pl, const double *pu, int nz,
-                                                         int cpr) {
-    __shared__ double lds[4][RB / 64];
+// One workgroup of WSB threads, four entries in flight per thread: the Re_tau=180
+// bands in the two-substep layout give 8192 (row, tile column) entries per layer.
+constexpr int WSB = 1024;
+__global__ __launch_bounds__(WSB) void k_fin_wshear_fused(WsArgs A, const double *pl, const double *pu, int nz,
+                                                          int cpr) {
+    __shared__ double lds[4][WSB / 64];
     double v[4] = {0., 0., 0., 0.};
     const int Yf = A.Yf;
     auto at = [&](const double *p, int m, int y, int c, int k) { return p[(((size_t)m * Yf + y) * cpr + c) * RED_W + k]; };
-    for (int e = threadIdx.x; e < Yf * cpr; e += RB) {
+#pragma unroll 4
+    for (int e = threadIdx.x; e < Yf * cpr; e += WSB) {
         const int y = e / cpr, c = e % cpr;
         v[0] += (-2. * at(pl, 0, y, c, 2) + 3. * at(pl, 1, y, c, 2) - at(pl, 2, y, c, 2));
         v[1] += at(pl, 0, y, c, 0);
         v[2] += -(2. * at(pu, nz - 1, y, c, 2) - 3. * at(pu, nz - 2, y, c, 2) + at(pu, nz - 3, y, c, 2));
         v[3] += at(pu, nz - 3, y, c, 0);
     }
-    block_sum<4>(v, lds);
+    block_sum<4, WSB>(v, lds);
     if (threadIdx.x == 0) {
         wshear_finish(A, v);
         if (A.enc && A.diag)
@@ -493,7 +497,7 @@ hipError_t launch_wshear_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec, con
     A.part = nullptr; A.Xf = h->d.Xf; A.Yf = h->d.Yf; A.NZs = h->d.NZs; A.GR = h->d.GR; A.omega_f = h->d.omega_f;
     A.force = h->d_force; A.diag = diag_rec; A.out = nullptr; A.enc = enc_rec;
     A.halt = h->halt_armed ? h->halt_flag : nullptr;
-    hipLaunchKernelGGL(k_fin_wshear_fused, dim3(1), dim3(RB), 0, s, A, pl, pu, h->lower.nz, cpr);
+    hipLaunchKernelGGL(k_fin_wshear_fused, dim3(1), dim3(WSB), 0, s, A, pl, pu, h->lower.nz, cpr);
     return hipGetLastError();
 }
 

@@ -160,6 +160,136 @@ __device__ __forceinline__ void cs_aligned_chunk(const LatArgs &L, const S *__re
     });
 }
 
+// Paired form (FP32 / FP16 storage, nx a multiple of 2 CS_THREADS): the workgroup
+// takes nodes x0 .. x0 + 2 CS_THREADS - 1, thread t the pair x = x0 + 2t, x + 1, read
+// and written as one 2-value access per direction.  As in the aligned form, the
+// workgroup also collides the nodes x0 - 1 and x0 + 2 CS_THREADS, and each thread
+// stores at (x, x + 1) the values arriving there from (x, x + 1) - e_x: for e_x = +1
+// the pair (f(x - 1), f(x)), f(x - 1) being the previous thread's second node; for
+// e_x = -1 the pair (f(x + 1), f(x + 2)).  At a wall end the pair's two positions
+// take the wall rules one value at a time.  The writes are exactly the scatter
+// form's.  rs[0] / rs[1] collect the sums of the first / second chunk's nodes.
+__device__ __forceinline__ void load_pair(const float *p, float &a, float &b) {
+    const float2 v = *reinterpret_cast<const float2 *>(p);
+    a = v.x;
+    b = v.y;
+}
+__device__ __forceinline__ void load_pair(const __half *p, float &a, float &b) {
+    const __half2 v = *reinterpret_cast<const __half2 *>(p);
+    a = __low2float(v);
+    b = __high2float(v);
+}
+__device__ __forceinline__ void store_pair2(float *p, float a, float b) {
+    *reinterpret_cast<float2 *>(p) = make_float2(a, b);
+}
+__device__ __forceinline__ void store_pair2(__half *p, float a, float b) {
+    *reinterpret_cast<__half2 *>(p) = __halves2half2(__float2half(a), __float2half(b));
+}
+
+template <class S, bool DS>
+__device__ __forceinline__ void cs_paired_chunk(const LatArgs &L, const S *__restrict__ cur, S *__restrict__ nxt,
+                                                int row, int x0, float G, float om, bool red, RowSums *rs) {
+    constexpr int NW = CS_THREADS / 64, NP = 2 * CS_THREADS;
+    __shared__ float hb[2][Q];      // post-collision nodes x0 - 1, x0 + NP
+    __shared__ float wb[NW][2][Q];  // per wave: last lane's second node e_x > 0, first lane's first node e_x < 0
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int y = row % ny, z = row / ny;
+    const bool lo_edge = (z == 0), hi_edge = (z == nz - 1);
+    const int ym = (y == 0) ? ny - 1 : y - 1, yp = (y == ny - 1) ? 0 : y + 1;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, x = x0 + 2 * t;
+    if (t == 0 || t == CS_THREADS - 1) {
+        const int side = t == 0 ? 0 : 1;
+        int xh = side == 0 ? x0 - 1 : x0 + NP;
+        xh = xh < 0 ? xh + nx : (xh >= nx ? xh - nx : xh);
+        float g[Q];
+        unroll<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            g[q] = load_s(cur + (size_t)q * nx + xh);
+        });
+        collide_any(g, G, om);
+        unroll<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (EX[q] != 0) hb[side][q] = g[q];
+        });
+    }
+    float f0[Q], f1[Q];
+    unroll<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        load_pair(cur + (size_t)q * nx + x, f0[q], f1[q]);
+    });
+    RowSums &rt = rs[t < CS_THREADS / 2 ? 0 : 1];
+    if (red) {
+        rt.add<S>(f0);
+        rt.add<S>(f1);
+    }
+    collide_any(f0, G, om);
+    collide_any(f1, G, om);
+    unroll<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (EX[q] > 0) {
+            if (lane == 63) wb[w][0][q] = f1[q];
+        } else if constexpr (EX[q] < 0) {
+            if (lane == 0) wb[w][1][q] = f0[q];
+        }
+    });
+    __syncthreads();
+    const bool ds = DS && L.dsum != nullptr;  // DS: compiled in only where a launch asks for it
+    const double wz[3] = {ds ? dest_w(L, z - 1) : 0., ds ? dest_w(L, z) : 0., ds ? dest_w(L, z + 1) : 0.};
+    unroll<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        // the values arriving at x and x + 1, and the sources they leave from
+        float va = f0[q], vb = f1[q];
+        if constexpr (EX[q] > 0) {
+            float s = __shfl_up(f1[q], 1, 64);
+            if (lane == 0) s = (w == 0) ? hb[0][q] : wb[w - 1][0][q];
+            va = s;
+            vb = f0[q];
+        } else if constexpr (EX[q] < 0) {
+            float s = __shfl_down(f0[q], 1, 64);
+            if (lane == 63) s = (w == NW - 1) ? hb[1][q] : wb[w + 1][1][q];
+            va = f1[q];
+            vb = s;
+        }
+        const int yt = (EY[q] > 0) ? yp : (EY[q] < 0 ? ym : y);
+        if constexpr (EZ[q] != 0) {
+            const bool at_end = (EZ[q] < 0) ? lo_edge : hi_edge;
+            if (at_end) {
+                const int kind = (EZ[q] < 0) ? L.lo_kind : L.hi_kind;
+                if (kind == BOUND_WALL) {
+                    const int8_t *wc = (EZ[q] < 0) ? L.wlo : L.whi;
+                    // positions x + i (i = 0, 1): the arriving value (source x + i - e_x)
+                    // is reflected specularly into (x + i, yt) unless it bounces back,
+                    // and the node's own value bounces back where its target is no-slip
+#pragma unroll
+                    for (int i = 0; i < 2; i++) {
+                        const int xi = x + i;
+                        const int xm = (xi == 0) ? nx - 1 : xi - 1, xp = (xi == nx - 1) ? 0 : xi + 1;
+                        const float v = i == 0 ? va : vb, own = i == 0 ? f0[q] : f1[q];
+                        const int xs = (EX[q] > 0) ? xm : (EX[q] < 0 ? xp : xi);
+                        if (!(wc[(size_t)yt * nx + xi] == -1 && wc[(size_t)y * nx + xs] == -1)) {
+                            store_s(nxt + lat_at(MIR[q], (size_t)z * ny + yt, xi, nx), v);
+                            if (ds) add_write(rt, wz[1], EX[q], read_back<S, q>(v));
+                        }
+                        const int xt = (EX[q] > 0) ? xp : (EX[q] < 0 ? xm : xi);
+                        if (wc[(size_t)yt * nx + xt] == -1 && wc[(size_t)y * nx + xi] == -1) {
+                            store_s(nxt + lat_at(OPP[q], (size_t)row, xi, nx), own);
+                            if (ds) add_write(rt, wz[1], -EX[q], read_back<S, q>(own));
+                        }
+                    }
+                }
+                return;
+            }
+        }
+        const int zt = z + EZ[q];
+        if ((zt == 0 && L.skip_lo) || (zt == nz - 1 && L.skip_hi)) return;
+        store_pair2(nxt + lat_at(q, (size_t)zt * ny + yt, x, nx), va, vb);
+        if (ds) {
+            add_write(rt, wz[EZ[q] + 1], EX[q], read_back<S, q>(va));
+            add_write(rt, wz[EZ[q] + 1], EX[q], read_back<S, q>(vb));
+        }
+    });
+}
+
 __device__ __forceinline__ void write_sums(double *red, size_t e, const RowSums &s) {
     red[e + 0] = s.rho;
     red[e + 1] = s.mom;
@@ -203,6 +333,26 @@ __global__ __launch_bounds__(CS_THREADS) void k_collide_stream(Batch<LatArgs> P)
     const bool red = L.red != nullptr;
     const int rpw = cs_rows_per_wg(nx);
     RowSums rs;
+    if constexpr (sizeof(S) < 8) {
+        if (L.paired) {
+            // two chunks of one row per workgroup, two nodes per thread
+            const int cpr = cs_chunks(nx), ppr = cpr / 2;
+            const int row = b / ppr, pc = b % ppr;
+            RowSums r2[2];
+            cs_paired_chunk<S, DS>(L, base + (size_t)row * Q * nx, nxt, row, pc * 2 * CS_THREADS, G, om, red, r2);
+            if (red || (DS && L.dsum)) {  // uniform per workgroup
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    if (c) __syncthreads();  // block_sums' shared partials are reused
+                    const RowSums t = block_sums(r2[c]);
+                    const size_t e = (size_t)row * cpr + 2 * pc + c;
+                    if (threadIdx.x == 0 && red) write_sums(L.red, e * RED_W, t);
+                    if (DS && threadIdx.x == 0 && L.dsum) { L.dsum[2 * e] = t.dm; L.dsum[2 * e + 1] = t.dd; }
+                }
+            }
+            return;
+        }
+    }
     if (rpw == 1) {
         // one (row, chunk) per workgroup: row, y and z are uniform over the workgroup
         const int cpr = cs_chunks(nx);
@@ -255,6 +405,7 @@ __global__ __launch_bounds__(CS_THREADS) void k_collide_stream(Batch<LatArgs> P)
 }
 
 hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, int prec, const int *halt) {
+    const int elem = prec == DNSG_PREC_FP64 ? 8 : (prec == DNSG_PREC_FP32 ? 4 : 2);
     for (int i0 = 0; i0 < nlat; i0 += BATCH_MAX) {
         Batch<LatArgs> P;
         P.halt = halt;
@@ -263,7 +414,7 @@ hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, i
         for (int i = 0; i < BATCH_MAX; i++) {
             P.it[i] = lats[i0 + (i < P.n ? i : 0)];
             P.first[i] = blocks;
-            if (i < P.n) blocks += cs_blocks(P.it[i].nx, P.it[i].ny * P.it[i].nz);
+            if (i < P.n) blocks += cs_blocks(P.it[i].nx, P.it[i].ny * P.it[i].nz, P.it[i].paired && elem < 8);
         }
         P.first[BATCH_MAX] = blocks;
         if (blocks == 0) continue;
