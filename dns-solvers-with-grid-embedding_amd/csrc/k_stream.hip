@@ -179,6 +179,14 @@ __device__ __forceinline__ void load_pair(const __half *p, float &a, float &b) {
     a = __low2float(v);
     b = __high2float(v);
 }
+__device__ __forceinline__ void load_pair(const double *p, double &a, double &b) {
+    const double2 v = *reinterpret_cast<const double2 *>(p);
+    a = v.x;
+    b = v.y;
+}
+__device__ __forceinline__ void store_pair2(double *p, double a, double b) {
+    *reinterpret_cast<double2 *>(p) = make_double2(a, b);
+}
 __device__ __forceinline__ void store_pair2(float *p, float a, float b) {
     *reinterpret_cast<float2 *>(p) = make_float2(a, b);
 }
@@ -186,12 +194,12 @@ __device__ __forceinline__ void store_pair2(__half *p, float a, float b) {
     *reinterpret_cast<__half2 *>(p) = __halves2half2(__float2half(a), __float2half(b));
 }
 
-template <class S, bool DS>
+template <class S, bool DS, class T>
 __device__ __forceinline__ void cs_paired_chunk(const LatArgs &L, const S *__restrict__ cur, S *__restrict__ nxt,
-                                                int row, int x0, float G, float om, bool red, RowSums *rs) {
+                                                int row, int x0, T G, T om, bool red, RowSums *rs) {
     constexpr int NW = CS_THREADS / 64, NP = 2 * CS_THREADS;
-    __shared__ float hb[2][Q];      // post-collision nodes x0 - 1, x0 + NP
-    __shared__ float wb[NW][2][Q];  // per wave: last lane's second node e_x > 0, first lane's first node e_x < 0
+    __shared__ T hb[2][Q];      // post-collision nodes x0 - 1, x0 + NP
+    __shared__ T wb[NW][2][Q];  // per wave: last lane's second node e_x > 0, first lane's first node e_x < 0
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const int y = row % ny, z = row / ny;
     const bool lo_edge = (z == 0), hi_edge = (z == nz - 1);
@@ -201,7 +209,7 @@ __device__ __forceinline__ void cs_paired_chunk(const LatArgs &L, const S *__res
         const int side = t == 0 ? 0 : 1;
         int xh = side == 0 ? x0 - 1 : x0 + NP;
         xh = xh < 0 ? xh + nx : (xh >= nx ? xh - nx : xh);
-        float g[Q];
+        T g[Q];
         unroll<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             g[q] = load_s(cur + (size_t)q * nx + xh);
@@ -212,7 +220,7 @@ __device__ __forceinline__ void cs_paired_chunk(const LatArgs &L, const S *__res
             if constexpr (EX[q] != 0) hb[side][q] = g[q];
         });
     }
-    float f0[Q], f1[Q];
+    T f0[Q], f1[Q];
     unroll<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         load_pair(cur + (size_t)q * nx + x, f0[q], f1[q]);
@@ -238,14 +246,14 @@ __device__ __forceinline__ void cs_paired_chunk(const LatArgs &L, const S *__res
     unroll<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         // the values arriving at x and x + 1, and the sources they leave from
-        float va = f0[q], vb = f1[q];
+        T va = f0[q], vb = f1[q];
         if constexpr (EX[q] > 0) {
-            float s = __shfl_up(f1[q], 1, 64);
+            T s = __shfl_up(f1[q], 1, 64);
             if (lane == 0) s = (w == 0) ? hb[0][q] : wb[w - 1][0][q];
             va = s;
             vb = f0[q];
         } else if constexpr (EX[q] < 0) {
-            float s = __shfl_down(f0[q], 1, 64);
+            T s = __shfl_down(f0[q], 1, 64);
             if (lane == 63) s = (w == NW - 1) ? hb[1][q] : wb[w + 1][1][q];
             va = f1[q];
             vb = s;
@@ -264,7 +272,7 @@ __device__ __forceinline__ void cs_paired_chunk(const LatArgs &L, const S *__res
                     for (int i = 0; i < 2; i++) {
                         const int xi = x + i;
                         const int xm = (xi == 0) ? nx - 1 : xi - 1, xp = (xi == nx - 1) ? 0 : xi + 1;
-                        const float v = i == 0 ? va : vb, own = i == 0 ? f0[q] : f1[q];
+                        const T v = i == 0 ? va : vb, own = i == 0 ? f0[q] : f1[q];
                         const int xs = (EX[q] > 0) ? xm : (EX[q] < 0 ? xp : xi);
                         if (!(wc[(size_t)yt * nx + xi] == -1 && wc[(size_t)y * nx + xs] == -1)) {
                             store_s(nxt + lat_at(MIR[q], (size_t)z * ny + yt, xi, nx), v);
@@ -333,13 +341,13 @@ __global__ __launch_bounds__(CS_THREADS) void k_collide_stream(Batch<LatArgs> P)
     const bool red = L.red != nullptr;
     const int rpw = cs_rows_per_wg(nx);
     RowSums rs;
-    if constexpr (sizeof(S) < 8) {
+    {
         if (L.paired) {
             // two chunks of one row per workgroup, two nodes per thread
             const int cpr = cs_chunks(nx), ppr = cpr / 2;
             const int row = b / ppr, pc = b % ppr;
             RowSums r2[2];
-            cs_paired_chunk<S, DS>(L, base + (size_t)row * Q * nx, nxt, row, pc * 2 * CS_THREADS, G, om, red, r2);
+            cs_paired_chunk<S, DS, T>(L, base + (size_t)row * Q * nx, nxt, row, pc * 2 * CS_THREADS, G, om, red, r2);
             if (red || (DS && L.dsum)) {  // uniform per workgroup
 #pragma unroll
                 for (int c = 0; c < 2; c++) {
@@ -405,7 +413,6 @@ __global__ __launch_bounds__(CS_THREADS) void k_collide_stream(Batch<LatArgs> P)
 }
 
 hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, int prec, const int *halt) {
-    const int elem = prec == DNSG_PREC_FP64 ? 8 : (prec == DNSG_PREC_FP32 ? 4 : 2);
     for (int i0 = 0; i0 < nlat; i0 += BATCH_MAX) {
         Batch<LatArgs> P;
         P.halt = halt;
@@ -414,7 +421,7 @@ hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, i
         for (int i = 0; i < BATCH_MAX; i++) {
             P.it[i] = lats[i0 + (i < P.n ? i : 0)];
             P.first[i] = blocks;
-            if (i < P.n) blocks += cs_blocks(P.it[i].nx, P.it[i].ny * P.it[i].nz, P.it[i].paired && elem < 8);
+            if (i < P.n) blocks += cs_blocks(P.it[i].nx, P.it[i].ny * P.it[i].nz, P.it[i].paired);
         }
         P.first[BATCH_MAX] = blocks;
         if (blocks == 0) continue;
