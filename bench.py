@@ -24,6 +24,7 @@ a Poisson stream, continuously batched, with p50/p95/p99 latency, throughput
 and an FP64 re-solve of a subset.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -160,20 +161,37 @@ def run_gpu(mod, prec, steps, warmup, force_mode="const"):
     return out
 
 
-def traffic_from_profiles():
-    """HBM bytes per band launch from the committed rocprofv3 --pmc summary, if any"""
-    path = os.path.join(REPO, "profiles", "pmc_band_collide_stream_retau180.txt")
-    if not os.path.exists(path):
-        return None
+TRAFFIC_PROFILES = {2: "pmc_band2_retau180_fp64.md",              # k_band2: both substeps in one launch
+                    1: "pmc_band_collide_stream_retau180.txt"}    # one collide-stream launch per substep
+
+
+def traffic_from_profiles(substeps):
+    """HBM bytes per band launch from the committed rocprofv3 --pmc summary of that
+    band kernel, and a note.  The summary lists the kernel's sources and their
+    SHA-256; a source that changed since makes the figure stale, and it is not used."""
+    path = os.path.join(REPO, "profiles", TRAFFIC_PROFILES.get(substeps, ""))
+    if not os.path.isfile(path):
+        return None, "no counter summary for this band kernel"
     vals = {}
     for line in open(path):
-        if "=" in line and not line.startswith("#"):
+        if "=" in line and not line.startswith("#") and not line.startswith("|"):
             k, v = line.split("=", 1)
             vals[k.strip()] = v.strip()
+    srcs = vals.get("sources", "").split()
+    if not srcs or "sources_sha256" not in vals:
+        return None, f"{os.path.basename(path)} names no source hash"
+    h = hashlib.sha256()
+    for f in srcs:
+        fp = os.path.join(PKG_DIR, f)
+        if not os.path.isfile(fp):
+            return None, f"{f} (listed in {os.path.basename(path)}) is missing"
+        h.update(open(fp, "rb").read())
+    if h.hexdigest() != vals["sources_sha256"]:
+        return None, f"stale: the sources listed in {os.path.basename(path)} changed since it was measured"
     try:
-        return float(vals["hbm_bytes_per_launch"])
+        return float(vals["hbm_bytes_per_launch"]), f"rocprofv3 FETCH_SIZE + WRITE_SIZE, {os.path.basename(path)}"
     except (KeyError, ValueError):
-        return None
+        return None, f"{os.path.basename(path)} has no hbm_bytes_per_launch"
 
 
 def cpu_baseline(mod, nsteps):
@@ -401,9 +419,9 @@ def main():
     main_t = res["fp64"] if "fp64" in res else res[tiers[0]]
     main_prec = "fp64" if "fp64" in res else tiers[0]
     bl = main_t["band_launch"]
-    traffic = traffic_from_profiles()
+    traffic, traffic_note = traffic_from_profiles(bl.get("substeps"))
     roof = dict(bound="hbm", achieved=bl["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=(bl["gbs"] / HBM_PEAK_GBS) if bl["gbs"] else None, traffic=traffic,
+                frac=(bl["gbs"] / HBM_PEAK_GBS) if bl["gbs"] else None, traffic=traffic, traffic_source=traffic_note,
                 kernel=("k_band2<double, false> (both wall bands, both gratio-2 substeps in one launch)"
                         if bl.get("substeps") == 2 else
                         "k_collide_stream<double, false> (both wall bands, one launch per substep)"),
