@@ -278,26 +278,27 @@ QUEUE_ENVELOPES = {"full": dict(x_range=(32, 256), y_range=(32, 256), z_range=(1
 QUEUE_STREAM = 256  # requests in the seeded stream; a run takes its first n
 
 
-def run_queue(mod, n, rate, seed, n_check, envelope="full", max_steps=400000):
-    """Slip-length serving queue: the first n requests of the seeded 256-request stream
+def run_queue(mod, n, rate, seed, n_check, envelope="full", max_steps=400000, first=0):
+    """Slip-length serving queue: requests first .. first+n-1 of the seeded 256-request stream
     (mod.slip_workload over the envelope: bands on a random half, stripes or posts, gas
     fraction 0.2..0.9, period 8..64 fine cells, shifted FP32), Poisson arrivals at `rate`
     per second, each solved until its bulk velocity moves by < 1e-9 (relative) over
     1000 steps (or, at the FP32 storage floor, stays within 3e-5 over 5 windows).
-    Latency = completion - arrival.  The first n_check requests are solved again in
+    Latency = completion - arrival.  The first n_check of them are solved again in
     FP64 (plain path, the 1e-9 criterion) and their slip lengths compared."""
     import random
     env = QUEUE_ENVELOPES[envelope]
-    work = mod.slip_workload(QUEUE_STREAM, seed, precision="fp32", **env)[:n]
+    work = mod.slip_workload(QUEUE_STREAM, seed, precision="fp32", **env)[first:first + n]
+    n = len(work)
     rng = random.Random(seed + 1)
     t, arrivals = 0.0, []
     for _ in range(n):
         t += rng.expovariate(rate)
         arrivals.append(t)
-    lu = {}
+    lu = []  # lattice updates per step of each request
     for d, p in work:
         dm = mod.dims_of(p)
-        lu[d["id"]] = dm.Xc * dm.Yc * dm.zdc + dm.GR * 2 * dm.Xf * dm.Yf * (dm.zdf - 1)
+        lu.append(dm.Xc * dm.Yc * dm.zdc + dm.GR * 2 * dm.Xf * dm.Yf * (dm.zdf - 1))
     q = mod.Queue(max_active=n, check_every=100, window=1000, tol=1e-9, max_steps=max_steps)
     t0 = q.clock()
     done, i, last = {}, 0, time.time()
@@ -322,7 +323,7 @@ def run_queue(mod, n, rate, seed, n_check, envelope="full", max_steps=400000):
     lat = np.array([done[k]["t_done"] - done[k]["t_arrival"] for k in range(n)])
     steps = np.array([done[k]["steps"] for k in range(n)])
     glu = sum(done[k]["steps"] * lu[k] for k in range(n)) / 1e9
-    out = dict(requests=n, stream=f"first {n} of the {QUEUE_STREAM}-request stream of seed {seed}",
+    out = dict(requests=n, stream=f"requests {first}..{first + n - 1} of the {QUEUE_STREAM}-request stream of seed {seed}",
                arrival_rate_per_s=rate, seed=seed, precision="fp32 shifted, fp64 reductions",
                envelope=dict(name=envelope, **env), converged_tol=int(sum(done[k]["status"] == 0 for k in range(n))),
                steady_at_fp32_floor=int(sum(done[k]["status"] == 2 for k in range(n))),
@@ -339,21 +340,27 @@ def run_queue(mod, n, rate, seed, n_check, envelope="full", max_steps=400000):
             d = work[k][0]
             q.submit(mod.slip_request(d["xdv"], d["ydv"], d["zdv"], d["gratio"], d["pattern"], d["period"],
                                       d["gas_fraction"], d["re_b"], precision="fp64"), k)
-        ref = {}
+        ref, last = {}, time.time()
         while len(ref) < n_check:
             for r in q.advance(1000):
                 ref[r["id"]] = r
+            if time.time() - last > 15:
+                print(f"bench: queue fp64 check done={len(ref)}/{n_check} {q.info()}", file=sys.stderr, flush=True)
+                last = time.time()
         q.close()
         db = [abs(done[k]["slip_length"] - ref[k]["slip_length"]) for k in range(n_check)]
         out["fp64_check"] = dict(requests=n_check, max_abs_diff_slip=float(max(db)),
                                  max_rel_diff_slip=float(max(db[k] / max(abs(ref[k]["slip_length"]), 1e-12)
                                                              for k in range(n_check))),
                                  fp64_converged_tol=int(sum(ref[k]["status"] == 0 for k in range(n_check))),
-                                 per_request=[dict(id=k, slip_fp32=done[k]["slip_length"],
+                                 per_request=[dict(id=first + k, slip_fp32=done[k]["slip_length"],
                                                    slip_fp64=ref[k]["slip_length"], status_fp32=done[k]["status"],
                                                    status_fp64=ref[k]["status"], steps_fp32=done[k]["steps"],
                                                    steps_fp64=ref[k]["steps"],
-                                                   latency_s=done[k]["t_done"] - done[k]["t_arrival"])
+                                                   latency_s=done[k]["t_done"] - done[k]["t_arrival"],
+                                                   lu_per_step=lu[k], **{f: work[k][0][f] for f in (
+                                                       "xdv", "ydv", "zdv", "gratio", "pattern", "period",
+                                                       "gas_fraction", "re_b")})
                                               for k in range(n_check)])
     return out
 
@@ -371,6 +378,7 @@ def main():
     ap.add_argument("--queue-rate", type=float, default=1.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--queue-check", type=int, default=4, help="requests re-solved in FP64 for the slip check")
     ap.add_argument("--queue-envelope", choices=sorted(QUEUE_ENVELOPES), default="full")
+    ap.add_argument("--queue-first", type=int, default=0, help="first request of the stream to serve")
     ap.add_argument("--queue-only", action="store_true", help="run the serving queue only and print its record")
     ap.add_argument("--no-queue", action="store_true")
     ap.add_argument("--no-precision-report", action="store_true")
@@ -396,7 +404,7 @@ def main():
 
     if args.queue_only:  # the serving workload alone: its record as one JSON line
         queue = run_queue(mod, args.queue_requests, args.queue_rate, 2025, min(args.queue_check, args.queue_requests),
-                          args.queue_envelope)
+                          args.queue_envelope, first=args.queue_first)
         print(json.dumps({"slip_queue": queue}), flush=True)
         return
 
