@@ -195,7 +195,7 @@ LatArgs coarse_args(dnsg_t *h) {
     L.red = nullptr;
     L.force = h->d_force;
     L.dsum = nullptr;
-    L.paired = h->cs_pair && cs_paired(L.nx, h->elem);
+    L.paired = h->cs_pair ? (cs_paired(L.nx, h->elem) ? 1 : 0) | (cs_paired(L.nx, h->elem, true) ? 2 : 0) : 0;
     set_dest_weights(h, 0, &L);
     return L;
 }
@@ -220,7 +220,7 @@ void band_args(dnsg_t *h, LatArgs *L) {
         L[b].red = nullptr;
         L[b].force = h->d_force;
         L[b].dsum = nullptr;
-        L[b].paired = h->cs_pair && cs_paired(g.nx, h->elem);
+        L[b].paired = h->cs_pair ? (cs_paired(g.nx, h->elem) ? 1 : 0) | (cs_paired(g.nx, h->elem, true) ? 2 : 0) : 0;
         set_dest_weights(h, 1 + b, &L[b]);
     }
 }

@@ -50,7 +50,7 @@ struct LatArgs {
     // klo..khi carry kscale (khalf: half of it), all others 0.
     double *dsum;
     int klo, khi, khalf;
-    int paired;  // the paired collide-stream form (cs_paired): two nodes per thread
+    int paired;  // the paired collide-stream form (cs_paired): bit 0 in a plain launch, bit 1 in a launch with dsum
     double kscale;
 };
 
@@ -92,12 +92,18 @@ __host__ __device__ __forceinline__ int cs_rows_per_wg(int nx) { return nx >= CS
 // nodes per thread, with 2-value loads and stores.  Measured on the Re_tau=180
 // lattices: FP32 coarse launch 0.956 -> 0.852 ms, FP16 0.599 -> 0.586 ms; FP64 with
 // pairs 0.97 ms against 0.94 ms in the aligned one-node form, so FP64 keeps that.
+// Launches that also sum their writes for the flux controller (LatArgs::dsum) use
+// DNSG_CS_PAIRED_DS_MASK: there FP64 pairs measured faster (the Re_tau=180 band
+// launch 1.032 -> 0.937 ms, the coarse one 1.936 -> 1.898 ms).
 #ifndef DNSG_CS_PAIRED_MASK
 #define DNSG_CS_PAIRED_MASK 6
 #endif
-__host__ __device__ __forceinline__ bool cs_paired(int nx, int elem) {
+#ifndef DNSG_CS_PAIRED_DS_MASK
+#define DNSG_CS_PAIRED_DS_MASK 7
+#endif
+__host__ __device__ __forceinline__ bool cs_paired(int nx, int elem, bool ds = false) {
     const int bit = elem == 8 ? 1 : (elem == 4 ? 2 : 4);
-    return (DNSG_CS_PAIRED_MASK & bit) != 0 && nx % (2 * CS_THREADS) == 0;
+    return ((ds ? DNSG_CS_PAIRED_DS_MASK : DNSG_CS_PAIRED_MASK) & bit) != 0 && nx % (2 * CS_THREADS) == 0;
 }
 // collide-stream workgroups of one lattice with `rows` rows of nx nodes
 __host__ __device__ __forceinline__ int cs_blocks(int nx, int rows, bool paired) {

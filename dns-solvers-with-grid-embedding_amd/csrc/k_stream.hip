@@ -417,16 +417,17 @@ hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, i
         Batch<LatArgs> P;
         P.halt = halt;
         P.n = nlat - i0 < BATCH_MAX ? nlat - i0 : BATCH_MAX;
+        bool ds = false;
+        for (int i = 0; i < P.n; i++) ds = ds || lats[i0 + i].dsum != nullptr;
         int blocks = 0;
         for (int i = 0; i < BATCH_MAX; i++) {
             P.it[i] = lats[i0 + (i < P.n ? i : 0)];
+            P.it[i].paired = (P.it[i].paired >> (ds ? 1 : 0)) & 1;  // the form for this launch
             P.first[i] = blocks;
             if (i < P.n) blocks += cs_blocks(P.it[i].nx, P.it[i].ny * P.it[i].nz, P.it[i].paired);
         }
         P.first[BATCH_MAX] = blocks;
         if (blocks == 0) continue;
-        bool ds = false;
-        for (int i = 0; i < P.n; i++) ds = ds || P.it[i].dsum != nullptr;
         const dim3 g(blocks), b(CS_THREADS);
         if (prec == DNSG_PREC_FP64) {
             if (ds) hipLaunchKernelGGL((k_collide_stream<double, true>), g, b, 0, s, P);

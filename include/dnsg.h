@@ -268,7 +268,14 @@ int dnsg_kernel_times(dnsg_t *h, int n, double *ms);
 
 /* Instrumentation (no counterpart in the reference): band substeps one band launch
  * of dnsg_step covers -- 2 where the handle runs both gratio-2 substeps in one pass
- * (the default at gratio 2), else 1. */
+ * (the default at gratio 2 in FP64 with constant force), else 1.  Read from the
+ * environment at dnsg_create (no counterpart in the reference; every choice gives
+ * the same state under constant force, and the same to the order of the reductions'
+ * sums under the flux controller): DNSG_BAND2=1 / 0 forces the one-pass band kernel on / off at
+ * gratio 2; DNSG_CS_PAIRED=0 keeps every collide-stream launch in its one-node form
+ * (by default FP32 / FP16 rows of a multiple of 512 nodes, and FP64 ones in the
+ * launches that sum their writes for the flux controller, take two nodes per
+ * thread). */
 int dnsg_band_launch_substeps(const dnsg_t *h, int *substeps);
 
 /* Instrumentation (no counterpart in the reference): achievable HBM bandwidth of
