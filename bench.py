@@ -162,7 +162,7 @@ def run_gpu(mod, prec, steps, warmup, force_mode="const"):
 
 
 TRAFFIC_PROFILES = {2: "pmc_band2_retau180_fp64.md",              # k_band2: both substeps in one launch
-                    1: "pmc_band_collide_stream_retau180.txt"}    # one collide-stream launch per substep
+                    1: "pmc_collide_stream_retau180_fp64.md"}    # one collide-stream launch per substep
 
 
 def traffic_from_profiles(substeps):
