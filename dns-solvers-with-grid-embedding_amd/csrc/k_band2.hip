@@ -135,9 +135,13 @@ __device__ __forceinline__ void band2_tile(const LatArgs &L, int tile, Lds<typen
         const int m = SIDE ? t : nz - 1 - t;
         const size_t row = (size_t)m * ny + y;
         if (t == 0) {  // the interface layer: the first fill's values
+            // the addresses are formed here, not kept from the kernel's start: they are
+            // loop-invariant, and holding 19 of them through the march spills registers
+            const S *fill = nxt + lat_at(0, row, x, nx);
+            asm volatile("" : "+v"(fill));
             unroll<0, Q>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                g[q] = act ? (T)load_s(nxt + lat_at(q, row, x, nx)) : rest_value<S>(q);
+                g[q] = act ? (T)load_s(fill + (size_t)q * nx) : rest_value<S>(q);
             });
         } else {
             const int sf = t & 1, su = (t - 1) % 3, sw = t % 3;
