@@ -412,13 +412,11 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
         bicubic_patch<GR>(xv, r);
     };
     // rho and u stay in registers for every direction of this workgroup
-    P dens[NP], u0[NP], u1[NP], u2[NP], vv[NP];
+    P dens[NP], u0[NP], u1[NP], u2[NP];
     phase(0, dens);
     phase(1, u0);
     phase(2, u1);
     phase(3, u2);
-#pragma unroll
-    for (int pp = 0; pp < NP; pp++) vv[pp] = u0[pp] * u0[pp] + u1[pp] * u1[pp] + u2[pp] * u2[pp];
     for (int iv = 4; iv < nv; iv++) {
         P r[NP];
         phase(iv, r);
@@ -429,7 +427,7 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
         // the compiler hoists it out of the loop for all 19 directions at once,
         // which holds 19 NP values in registers and lowers occupancy
 #pragma unroll
-        for (int pp = 0; pp < NP; pp++) asm volatile("" : "+v"(dens[pp]), "+v"(u0[pp]), "+v"(u1[pp]), "+v"(u2[pp]), "+v"(vv[pp]));
+        for (int pp = 0; pp < NP; pp++) asm volatile("" : "+v"(dens[pp]), "+v"(u0[pp]), "+v"(u1[pp]), "+v"(u2[pp]));
         unroll<0, Q>([&](auto qc) {
             constexpr int QD = decltype(qc)::value;
             if (q != QD) return;
@@ -438,12 +436,15 @@ __global__ __launch_bounds__(TX *TY) void k_iface(Batch<IfaceItem> BT, HermiteW<
 #pragma unroll
             for (int pp = 0; pp < NP; pp++) {
                 const P eu = edot<QD>(u0[pp], u1[pp], u2[pp]);
+                // u.u formed again for every direction, the same expression each time,
+                // rather than held through the loop (registers)
+                const P vsq = u0[pp] * u0[pp] + u1[pp] * u1[pp] + u2[pp] * u2[pp];
                 if constexpr (sizeof(S) == 8) {
-                    const P feq = wq * dens[pp] * ((P)1. + (P)3. * eu + (P)4.5 * eu * eu - (P)1.5 * vv[pp]);
+                    const P feq = wq * dens[pp] * ((P)1. + (P)3. * eu + (P)4.5 * eu * eu - (P)1.5 * vsq);
                     out[pp] = feq + rescale * r[pp];  // ctfdtransfer_opt.c:93
                 } else {  // dens holds rho - 1
                     const P rho = (P)1. + dens[pp];
-                    const P geq = wq * (dens[pp] + rho * ((P)3. * eu + (P)4.5 * eu * eu - (P)1.5 * vv[pp]));
+                    const P geq = wq * (dens[pp] + rho * ((P)3. * eu + (P)4.5 * eu * eu - (P)1.5 * vsq));
                     out[pp] = geq + rescale * r[pp];
                 }
             }
