@@ -1344,11 +1344,8 @@ void orc_profile2d(const orc_t *o, double *jj, double *rho) {
 }
 
 /* ---- one coarse step: main:1256-1375 (file I/O excluded) ---------------------------- */
-int orc_step(orc_t *o, long ss, dnsg_diag *dg) {
-    int iter, rc;
-    orc_encalc2(o, ss, dg);                 /* :1257 */
-    rc = orc_wshearstr(o, ss, dg);          /* :1258 */
-    if (rc) return rc;
+static void step_after_reductions(orc_t *o) {
+    int iter;
     orc_xcomputations(o);                   /* :1345 */
     for (iter = 1; iter < o->GR; iter++) {  /* :1347-1358 */
         orc_xfcomputations(o);
@@ -1366,6 +1363,29 @@ int orc_step(orc_t *o, long ss, dnsg_diag *dg) {
     orc_ftcdtransfer(o);                    /* :1366-1368 */
     orc_exchange(o);                        /* :1373-1374 */
     orc_fexchange(o);
+}
+
+int orc_step(orc_t *o, long ss, dnsg_diag *dg) {
+    int rc;
+    orc_encalc2(o, ss, dg);                 /* :1257 */
+    rc = orc_wshearstr(o, ss, dg);          /* :1258 */
+    if (rc) return rc;
+    step_after_reductions(o);
+    return 0;
+}
+
+/* The same step with the body force that encalc2 forms replaced by g.  Test hook: the
+   constant-force mode given, step by step, the force the flux controller formed must
+   reproduce the controller's run, so that the two modes differ only in where the force
+   comes from (encalc2.c:129 against a constant). */
+int orc_step_forced(orc_t *o, long ss, dnsg_diag *dg, double g) {
+    int rc;
+    orc_encalc2(o, ss, dg);
+    o->force[0] = g;
+    if (dg) dg->force = g;
+    rc = orc_wshearstr(o, ss, dg);
+    if (rc) return rc;
+    step_after_reductions(o);
     return 0;
 }
 

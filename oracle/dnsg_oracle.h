@@ -84,6 +84,8 @@ void orc_exchange(orc_t *o);
 void orc_fexchange(orc_t *o);
 /* one full coarse step (main:1256-1375 minus file I/O); returns -3 on NaN stress */
 int  orc_step(orc_t *o, long ss, dnsg_diag *d);
+/* the same step with the body force replaced by g once encalc2 has formed it (tests) */
+int  orc_step_forced(orc_t *o, long ss, dnsg_diag *d, double g);
 
 /* reductions */
 double orc_mass(const orc_t *o);                 /* main:1193-1250 */

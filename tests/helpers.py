@@ -47,6 +47,7 @@ def oracle_lib():
                   "orc_ftcdtransfer", "orc_exchange", "orc_fexchange"):
             getattr(lib, n).argtypes = [ctypes.c_void_p]
         lib.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.POINTER(dnsg.Diag)]
+        lib.orc_step_forced.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.POINTER(dnsg.Diag), ctypes.c_double]
         lib.orc_encalc2.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.POINTER(dnsg.Diag)]
         lib.orc_wshearstr.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.POINTER(dnsg.Diag)]
         lib.orc_tinterpolate_hrmt.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -152,6 +153,12 @@ class Oracle:
     def step(self, ss):
         d = dnsg.Diag()
         rc = self.lib.orc_step(self.o, ss, ctypes.byref(d))
+        return rc, d
+
+    def step_forced(self, ss, g):
+        """one step with the body force replaced by g after encalc2 forms it"""
+        d = dnsg.Diag()
+        rc = self.lib.orc_step_forced(self.o, ss, ctypes.byref(d), g)
         return rc, d
 
     def run(self, n, ss0=0):

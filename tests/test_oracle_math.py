@@ -236,6 +236,35 @@ def _plug_flow(o, U):
     return f
 
 
+def test_constant_force_mode_is_the_controller_path(oracle_built):
+    """The constant-force mode (the bitwise GPU cases) and the reference's flux
+    controller differ only in where the body force comes from: encalc2.c:129 forms
+    G = ub - <rho u>, the constant mode takes g_const.  The controller's run is pinned
+    byte for byte to the reference's own files (test_oracle_golden.py).  Given, step by
+    step, the G the controller formed, the constant-force oracle reproduces that run bit
+    for bit, walls, interface and injection included; a G one ulp off at one step does
+    not."""
+    kw = dict(MICRO, pattern="rslip", gyplussize=4, wyplussize=4, init_kind="laminar_noise", seed=7,
+              u_max=0.05, noise=0.05)
+    flux = Oracle(dnsg.make_params(**kw))
+    const = Oracle(dnsg.make_params(**kw, force_mode="const", g_const=3e-5))
+    bent = Oracle(dnsg.make_params(**kw, force_mode="const", g_const=3e-5))
+    for o in (flux, const, bent):
+        o.init_seeded()
+    gs = []
+    for s in range(12):
+        rc, d = flux.step(s)
+        assert rc == 0
+        gs.append(d.force)
+        assert const.step_forced(s, d.force)[0] == 0
+        assert bent.step_forced(s, np.nextafter(d.force, np.inf) if s == 5 else d.force)[0] == 0
+    assert len(set(gs)) > 1  # the controller moved the force
+    a, b, c = flux.state(), const.state(), bent.state()
+    for k in ("coarse", "lower", "upper", "lin", "uin"):
+        assert np.array_equal(a[k], b[k]), k
+    assert any(not np.array_equal(a[k], c[k]) for k in ("coarse", "lower", "upper"))
+
+
 def test_specular_walls_keep_plug_flow_exact(oracle_built):
     """all-gas walls transmit no shear: uniform plug flow is an exact steady state,
     while the same flow between no-slip (bounce-back) walls is decelerated"""
