@@ -868,15 +868,17 @@ int dnsg::create_ctx(const dnsg_params *p, hipStream_t ext, dnsg_t **out) {
     h->elem = p->precision == DNSG_PREC_FP64 ? 8 : (p->precision == DNSG_PREC_FP32 ? 4 : 2);
     {
         // gratio 2: both band substeps in one pass (k_band2.hip), by default in the FP64
-        // constant-force configuration, where it is the faster path; DNSG_BAND2=1 /
+        // and FP32 constant-force configurations, where it is the faster path (FP16
+        // storage and the flux controller: one launch per substep); DNSG_BAND2=1 /
         // DNSG_BAND2=0 in the environment selects it / the one-launch-per-substep path
-        // for any configuration (same results)
+        // for any configuration (the same state; the reductions' partials differ)
         // FP32 / FP16 rows of a multiple of 2 CS_THREADS nodes: the paired collide-stream
         // form (k_stream.hip), unless DNSG_CS_PAIRED=0 (the scatter form; same state)
         const char *cp = getenv("DNSG_CS_PAIRED");
         h->cs_pair = !(cp && cp[0] == '0');
         const char *e = getenv("DNSG_BAND2");
-        const bool pick = p->precision == DNSG_PREC_FP64 && p->force_mode == DNSG_FORCE_CONST;
+        const bool pick = (p->precision == DNSG_PREC_FP64 || p->precision == DNSG_PREC_FP32) &&
+                          p->force_mode == DNSG_FORCE_CONST;
         h->band2 = h->d.GR == 2 && ((e && e[0]) ? e[0] == '1' : pick);
     }
     const dnsg_dims &d = h->d;

@@ -232,7 +232,7 @@ hipError_t launch_collide_stream(hipStream_t s, const LatArgs *lats, int nlat, i
 // must already be in their nxt lattices' interface layers; red / dsum in the B2_TX
 // partial layout
 hipError_t launch_band2(hipStream_t s, const LatArgs *bands, int prec, const int *halt = nullptr);
-int band2_tiles(int nx, int ny);
+int band2_tiles(int nx, int ny, int prec);
 hipError_t launch_init_lattice(hipStream_t s, dnsg_ctx *h, int grid, void *dst);
 // gl = 1: the host array carries one ghost x-plane and y-row (the reference's in-memory
 // layout); downloads write them as periodic images

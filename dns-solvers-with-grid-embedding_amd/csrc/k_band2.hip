@@ -9,7 +9,8 @@
 // fill reads only the interface planes.  So the fill runs first, into the output
 // lattice's interface layer, and one workgroup then carries a tile of the band
 // through both substeps, keeping the intermediate state on chip:
-//   * a tile is 64 x-consecutive columns (one wave per row) by 4 rows, all layers;
+//   * a tile is 64 x-consecutive columns (one wave per row) by 4 rows (FP64) or 8
+//     rows (FP32, FP16), all layers;
 //   * the workgroup marches through the layers from the interface towards the wall.
 //     At layer t it loads and collides the first substep's input on the tile and a
 //     one-node ring around it (the rows above and below, one wave each, and the
@@ -45,10 +46,18 @@ namespace dnsg {
 namespace {
 
 constexpr int TX = B2_TX;          // columns per tile = lanes per row wave
-constexpr int TY = 4;              // output rows per tile
-constexpr int NR = TY + 2;         // row waves: the ring row above, TY rows, the ring row below
-constexpr int NT = 64 * (NR + 1);  // + one wave for the ring columns
 constexpr int CW = TX + 2;         // LDS columns: left ring column, the tile's 64, right ring column
+// Output rows per tile.  The workgroup has a row wave per tile row, one for the ring
+// row above and one for the ring row below, and one wave for the ring columns; a CU
+// holds one workgroup (LDS).  FP64 needs 240 VGPRs, two waves per SIMD: 4 rows, 7
+// waves.  FP32 and FP16 need 160, three waves per SIMD: 8 rows, 11 waves (9 rows,
+// 12 waves, measured slower).
+template <class S>
+constexpr int tile_rows() { return sizeof(S) == 8 ? 4 : 8; }
+template <class S>
+constexpr int row_waves() { return tile_rows<S>() + 2; }
+template <class S>
+constexpr int wg_threads() { return 64 * (row_waves<S>() + 1); }
 
 // March coordinate t: 0 at the interface layer, nz-1 next to the wall.  A direction
 // moves t by ez for the upper band (interface = stored layer 0) and by -ez for the
@@ -72,7 +81,7 @@ constexpr int yi(int q) {
 // along the layer t & 1 (read one iteration later), toward the wall t % 3 (read two
 // iterations later), toward the interface one (read in the same iteration); two
 // barriers per iteration separate the writes of a layer from the reads of older ones.
-template <class T>
+template <class T, int NR>
 struct Lds {
     T yf[2][6][NR][CW];
     T yu[3][2][NR][CW];
@@ -99,8 +108,10 @@ __device__ __forceinline__ typename ComputeOf<S>::T rest_value(int q) {
 }
 
 template <class S, int SIDE, bool DS>
-__device__ __forceinline__ void band2_tile(const LatArgs &L, int tile, Lds<typename ComputeOf<S>::T> &sm) {
+__device__ __forceinline__ void band2_tile(const LatArgs &L, int tile,
+                                           Lds<typename ComputeOf<S>::T, row_waves<S>()> &sm) {
     using T = typename ComputeOf<S>::T;
+    constexpr int TY = tile_rows<S>(), NR = row_waves<S>();
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const int tilesx = (nx + TX - 1) / TX;
     const int tx = tile % tilesx, ty = tile / tilesx;
@@ -267,6 +278,14 @@ __device__ __forceinline__ void band2_tile(const LatArgs &L, int tile, Lds<typen
                 }
             }
             collide_any(f, G, om);
+            if constexpr (sizeof(S) == 2) {
+                // FP16 storage: one launch per substep stores this result in FP16 and the
+                // second substep reads it back; round it the same way here
+                unroll<0, Q>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    f[q] = __half2float(__float2half(f[q]));
+                });
+            }
             const int sf = t & 1, su = t % 3;
             unroll<0, Q>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
@@ -328,9 +347,9 @@ __device__ __forceinline__ void band2_tile(const LatArgs &L, int tile, Lds<typen
 }
 
 template <class S, bool DS>
-__global__ __launch_bounds__(NT) void k_band2(Batch<LatArgs> P) {
+__global__ __launch_bounds__(wg_threads<S>()) void k_band2(Batch<LatArgs> P) {
     using T = typename ComputeOf<S>::T;
-    __shared__ Lds<T> sm;
+    __shared__ Lds<T, row_waves<S>()> sm;
     if (halted(P.halt)) return;
     const int bg = xcd_remap(blockIdx.x, gridDim.x);
     const int it = batch_find(P.first, P.n, bg);
@@ -342,7 +361,10 @@ __global__ __launch_bounds__(NT) void k_band2(Batch<LatArgs> P) {
 
 }  // namespace
 
-int band2_tiles(int nx, int ny) { return ((nx + TX - 1) / TX) * ((ny + TY - 1) / TY); }
+int band2_tiles(int nx, int ny, int prec) {
+    const int ty = prec == DNSG_PREC_FP64 ? tile_rows<double>() : tile_rows<float>();
+    return ((nx + TX - 1) / TX) * ((ny + ty - 1) / ty);
+}
 
 hipError_t launch_band2(hipStream_t s, const LatArgs *bands, int prec, const int *halt) {
     Batch<LatArgs> P;
@@ -352,11 +374,11 @@ hipError_t launch_band2(hipStream_t s, const LatArgs *bands, int prec, const int
     for (int i = 0; i < BATCH_MAX; i++) {
         P.it[i] = bands[i < 2 ? i : 0];
         P.first[i] = blocks;
-        if (i < 2) blocks += band2_tiles(bands[i].nx, bands[i].ny);
+        if (i < 2) blocks += band2_tiles(bands[i].nx, bands[i].ny, prec);
     }
     P.first[BATCH_MAX] = blocks;
     const bool ds = bands[0].dsum != nullptr || bands[1].dsum != nullptr;
-    const dim3 g(blocks), b(NT);
+    const dim3 g(blocks), b(prec == DNSG_PREC_FP64 ? wg_threads<double>() : wg_threads<float>());
     if (prec == DNSG_PREC_FP64) {
         if (ds) hipLaunchKernelGGL((k_band2<double, true>), g, b, 0, s, P);
         else hipLaunchKernelGGL((k_band2<double, false>), g, b, 0, s, P);

@@ -268,7 +268,7 @@ int dnsg_kernel_times(dnsg_t *h, int n, double *ms);
 
 /* Instrumentation (no counterpart in the reference): band substeps one band launch
  * of dnsg_step covers -- 2 where the handle runs both gratio-2 substeps in one pass
- * (the default at gratio 2 in FP64 with constant force), else 1.  Read from the
+ * (the default at gratio 2 in FP64 and FP32 with constant force), else 1.  Read from the
  * environment at dnsg_create (no counterpart in the reference; every choice gives
  * the same state under constant force, and the same to the order of the reductions'
  * sums under the flux controller): DNSG_BAND2=1 / 0 forces the one-pass band kernel on / off at
