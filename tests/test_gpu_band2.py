@@ -66,3 +66,38 @@ def test_band2_equals_two_launches_flux_control(prec, monkeypatch):
         assert relmax(a[k], b[k]) <= tol, (k, relmax(a[k], b[k]))
     assert relmax(da["flowrate"], db["flowrate"]) <= tol
     assert np.max(np.abs(da["force"] - db["force"])) <= tol * kw["ub"]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+def test_band2_nan_stops_where_two_launches_stop(prec, monkeypatch):
+    """A NaN planted in the middle of the coarse grid reaches the bands' wall layers a
+    few steps later.  Asked for many more steps in one call, both band paths return
+    the NaN status at the same step, with the same records up to it, and hold the same
+    state: that step's top (the band pass swaps the band buffers once per step, the
+    two-launch path twice, and each rewinds by its own count)."""
+    kw = CASES["x40_y6_pslip"]
+    out = {}
+    for band2 in (True, False):
+        monkeypatch.setenv("DNSG_BAND2", "1" if band2 else "0")
+        s = dnsg.Solver(dnsg.make_params(**kw, precision=prec, force_mode="const", g_const=2e-5))
+        s.init_state()
+        st = s.download()
+        st["coarse"][20, 3, 6, 0] = np.nan
+        s.upload(st["coarse"], st["lower"], st["upper"], st["lin"], st["uin"], st["force"])
+        n = 60
+        d = (dnsg.Diag * n)()
+        rc = s.lib.dnsg_step(s.h, n, d)
+        msg = s.lib.dnsg_last_error(s.h).decode()
+        print(f"{prec} band2={band2}: rc {rc}, {msg}", flush=True)
+        assert rc == -3
+        fail = int(msg.rsplit(" ", 1)[1])
+        out[band2] = (fail, [(d[k].time, d[k].force, d[k].stress) for k in range(fail + 1)], s.download())
+        s.close()
+    (fa, ra, ga), (fb, rb, gb) = out[True], out[False]
+    assert fa == fb and fa >= 1
+    for (ta, fca, sa), (tb, fcb, sb) in zip(ra[:-1], rb[:-1]):
+        assert ta == tb and fca == fcb
+        assert relmax(np.array([sa]), np.array([sb])) <= 1e-10
+    assert np.isnan(ra[-1][2]) and np.isnan(rb[-1][2])
+    for k in GRIDS + ("lin", "uin"):
+        assert np.array_equal(ga[k], gb[k], equal_nan=True), k
