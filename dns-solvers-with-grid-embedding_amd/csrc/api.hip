@@ -360,8 +360,8 @@ int do_step_band2(dnsg_t *h, double *rec, SecTimer *T) {
     HIPCHK(h, launch_band2(h->stream, B, h->p.precision, h->halt()));
     if (T) T->end(h->stream, SEC_BAND);
     if (T) T->begin(h->stream, SEC_REDUCE);
-    if (!flux) HIPCHK(h, launch_encalc2_fused(h->stream, h, rec));
-    HIPCHK(h, launch_wshear_fused(h->stream, h, rec, from_writes ? h->enc_next : nullptr));
+    if (!flux) HIPCHK(h, launch_reductions_fused(h->stream, h, rec));
+    else HIPCHK(h, launch_wshear_fused(h->stream, h, rec, from_writes ? h->enc_next : nullptr));
     if (T) T->end(h->stream, SEC_REDUCE);
     if (T) T->begin(h->stream, SEC_IFACE);
     HIPCHK(h, launch_interface(h->stream, h, 2));
@@ -424,8 +424,8 @@ int do_step(dnsg_t *h, double *rec, SecTimer *T = nullptr) {
             // the reductions first: a NaN wall stress halts every later launch, the
             // sampling included, so the state stays that at the top of this step
             if (T) T->begin(h->stream, SEC_REDUCE);
-            if (!flux) HIPCHK(h, launch_encalc2_fused(h->stream, h, rec));
-            HIPCHK(h, launch_wshear_fused(h->stream, h, rec, from_writes ? h->enc_next : nullptr));
+            if (!flux) HIPCHK(h, launch_reductions_fused(h->stream, h, rec));
+            else HIPCHK(h, launch_wshear_fused(h->stream, h, rec, from_writes ? h->enc_next : nullptr));
             if (T) T->end(h->stream, SEC_REDUCE);
             rotate_levels(h);
             if (T) T->begin(h->stream, SEC_SAMPLE);

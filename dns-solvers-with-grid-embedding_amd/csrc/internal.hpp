@@ -270,8 +270,10 @@ int ftcd_blocks(const dnsg_ctx *h);
 // out[2 slot[i]] = <rho u_x>, out[2 slot[i] + 1] = <rho> (encalc2.c:109-131 weights)
 hipError_t launch_flux_many(hipStream_t s, dnsg_ctx *const *hs, int n, double *out, const int *slot);
 hipError_t launch_encalc2(hipStream_t s, dnsg_ctx *h, double *diag_rec);
-hipError_t launch_encalc2_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec);
 hipError_t launch_wshear_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec, const double *enc_rec = nullptr);
+// constant force: encalc2 from the collide-stream kernels' row sums (RED_W per row) and
+// wshearstr from the bands', in two launches with a shared first stage
+hipError_t launch_reductions_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec);
 // G of the next step from the destination-weighted sums of this step's writes
 hipError_t launch_flux_next(hipStream_t s, dnsg_ctx *h);
 size_t dsum_doubles(const dnsg_ctx *h);

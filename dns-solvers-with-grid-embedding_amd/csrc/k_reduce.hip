@@ -199,26 +199,29 @@ __global__ __launch_bounds__(RB) void k_enc_seg(EncArgs A, double *seg) {
         for (int k = 0; k < 4; k++) seg[blockIdx.x * 4 + k] = v[k];
 }
 
+// encalc2's force and log values from the sums (momentum, denst, cmomentum, cdenst)
+__device__ void enc_finish(const EncArgs &A, const double *v) {
+    const double vol = ((double)((A.Xf + 1 - 1.) * (A.Yf + 1 - 1.) * (A.NZs - 2.)));  // encalc2.c:109
+    const double momentum = v[0] / vol, denst = v[1] / vol;
+    const double G = (A.force_mode == DNSG_FORCE_FLUX) ? A.ub - momentum : A.g_const;  // :129
+    A.force[0] = G;
+    A.force[1] = v[2] / vol;
+    A.force[2] = v[3] / vol;
+    if (A.diag) {
+        A.diag[1] = G;
+        A.diag[2] = momentum;
+        A.diag[3] = denst;
+        A.diag[4] = momentum * ((A.Yf) * (A.NZs - 2)) / denst;  // :138
+    }
+}
+
 __global__ __launch_bounds__(RB) void k_fin_encalc2(EncArgs A, const double *seg, int nseg) {
     __shared__ double lds[4][RB / 64];
     double v[4] = {0., 0., 0., 0.};  // momentum, denst, cmomentum, cdenst
     for (int b = threadIdx.x; b < nseg; b += RB)
         for (int k = 0; k < 4; k++) v[k] += seg[b * 4 + k];
     block_sum<4>(v, lds);
-    if (threadIdx.x == 0) {
-        const double vol = ((double)((A.Xf + 1 - 1.) * (A.Yf + 1 - 1.) * (A.NZs - 2.)));  // encalc2.c:109
-        const double momentum = v[0] / vol, denst = v[1] / vol;
-        const double G = (A.force_mode == DNSG_FORCE_FLUX) ? A.ub - momentum : A.g_const;  // :129
-        A.force[0] = G;
-        A.force[1] = v[2] / vol;
-        A.force[2] = v[3] / vol;
-        if (A.diag) {
-            A.diag[1] = G;
-            A.diag[2] = momentum;
-            A.diag[3] = denst;
-            A.diag[4] = momentum * ((A.Yf) * (A.NZs - 2)) / denst;  // :138
-        }
-    }
+    if (threadIdx.x == 0) enc_finish(A, v);
 }
 
 static hipError_t enc_two_stage(hipStream_t s, const EncArgs &A, double *seg) {
@@ -229,10 +232,13 @@ static hipError_t enc_two_stage(hipStream_t s, const EncArgs &A, double *seg) {
     return hipGetLastError();
 }
 
+constexpr int WS_SEG = 512;  // wall-stress entries per first-stage workgroup (launch_reductions_fused)
+
 size_t enc_seg_doubles(const dnsg_ctx *h) {
     const size_t total = (size_t)h->d.Yc * h->d.zdc * cs_chunks(h->d.Xc) +
                          2 * (size_t)h->d.Yf * (h->d.zdf - 1) * band_cpr_max(h) + (size_t)ftcd_blocks(h);
-    return 4 * (total / ENC_SEG + 2);
+    const size_t ws = (size_t)h->d.Yf * band_cpr_max(h);
+    return 4 * (total / ENC_SEG + 2 + ws / WS_SEG + 1);
 }
 
 // ---- flux controller from the writes of a step ------------------------------------------
@@ -335,14 +341,6 @@ static EncArgs enc_args(dnsg_ctx *h, const double *pc, const double *pl, const d
     A.NZs = h->d.NZs; A.Xf = h->d.Xf; A.force_mode = h->p.force_mode; A.ub = h->p.ub; A.g_const = h->p.g_const;
     A.force = h->d_force; A.diag = diag_rec;
     return A;
-}
-
-// encalc2 from the row sums the collide-stream kernels wrote (RED_W per row)
-hipError_t launch_encalc2_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec) {
-    const double *pc = h->red, *pl = pc + (size_t)h->coarse.ny * h->coarse.nz * cs_chunks(h->coarse.nx) * RED_W;
-    const double *pu = pl + (size_t)h->lower.ny * h->lower.nz * band_cpr(h) * RED_W;
-    EncArgs A = enc_args(h, pc, pl, pu, RED_W, true, diag_rec);
-    return enc_two_stage(s, A, h->seg);
 }
 
 // Bulk momentum and density of many solves at once: one workgroup per solve sums
@@ -498,6 +496,75 @@ hipError_t launch_wshear_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec, con
     A.force = h->d_force; A.diag = diag_rec; A.out = nullptr; A.enc = enc_rec;
     A.halt = h->halt_armed ? h->halt_flag : nullptr;
     hipLaunchKernelGGL(k_fin_wshear_fused, dim3(1), dim3(WSB), 0, s, A, pl, pu, h->lower.nz, cpr);
+    return hipGetLastError();
+}
+
+// ---- encalc2 and wshearstr of one step together (constant force) ---------------------------
+// Their first stages run as one launch: workgroups [0, nseg) sum encalc2's entries as
+// k_enc_seg does, workgroups [nseg, nseg + nws) WS_SEG (row, tile column) entries each
+// of the wall stress as k_fin_wshear_fused does.  One workgroup then finishes encalc2
+// and, with the force that sets, wshearstr -- two launches where the separate
+// reductions take three, one of them a single workgroup over all the wall-stress
+// entries.
+struct WsSegArgs {
+    const double *pl, *pu;  // the bands' fused-reduction partials
+    int Yf, cpr, nz;
+};
+
+__device__ __forceinline__ void ws_entry(const WsSegArgs &W, int e, double *v) {
+    const int y = e / W.cpr, c = e % W.cpr;
+    auto at = [&](const double *p, int m, int k) { return p[(((size_t)m * W.Yf + y) * W.cpr + c) * RED_W + k]; };
+    v[0] += (-2. * at(W.pl, 0, 2) + 3. * at(W.pl, 1, 2) - at(W.pl, 2, 2));
+    v[1] += at(W.pl, 0, 0);
+    v[2] += -(2. * at(W.pu, W.nz - 1, 2) - 3. * at(W.pu, W.nz - 2, 2) + at(W.pu, W.nz - 3, 2));
+    v[3] += at(W.pu, W.nz - 3, 0);
+}
+
+__global__ __launch_bounds__(RB) void k_red_seg(EncArgs A, WsSegArgs W, double *seg, int nseg) {
+    __shared__ double lds[4][RB / 64];
+    double v[4] = {0., 0., 0., 0.};
+    if ((int)blockIdx.x < nseg) {  // uniform per workgroup
+        const EncSpace S = enc_space(A);
+        const int e0 = blockIdx.x * ENC_SEG, e1 = min(S.total(), e0 + ENC_SEG);
+        for (int e = e0 + threadIdx.x; e < e1; e += RB) enc_entry(A, S, e, v);
+    } else {
+        const int e0 = (blockIdx.x - nseg) * WS_SEG, e1 = min(W.Yf * W.cpr, e0 + WS_SEG);
+        for (int e = e0 + threadIdx.x; e < e1; e += RB) ws_entry(W, e, v);
+    }
+    block_sum<4>(v, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; k++) seg[blockIdx.x * 4 + k] = v[k];
+}
+
+__global__ __launch_bounds__(RB) void k_fin_enc_ws(EncArgs A, WsArgs Wf, const double *seg, int nseg, int nws) {
+    __shared__ double lds[4][RB / 64];
+    double v[4] = {0., 0., 0., 0.};
+    for (int b = threadIdx.x; b < nseg; b += RB)
+        for (int k = 0; k < 4; k++) v[k] += seg[b * 4 + k];
+    block_sum<4>(v, lds);
+    if (threadIdx.x == 0) enc_finish(A, v);  // writes the force that wshear_finish reads
+    double w[4] = {0., 0., 0., 0.};
+    for (int b = threadIdx.x; b < nws; b += RB)
+        for (int k = 0; k < 4; k++) w[k] += seg[(nseg + b) * 4 + k];
+    block_sum<4>(w, lds);
+    if (threadIdx.x == 0) wshear_finish(Wf, w);
+}
+
+hipError_t launch_reductions_fused(hipStream_t s, dnsg_ctx *h, double *diag_rec) {
+    const int cpr = band_cpr(h);
+    const double *pc = h->red, *pl = pc + (size_t)h->coarse.ny * h->coarse.nz * cs_chunks(h->coarse.nx) * RED_W;
+    const double *pu = pl + (size_t)h->lower.ny * h->lower.nz * cpr * RED_W;
+    const EncArgs A = enc_args(h, pc, pl, pu, RED_W, true, diag_rec);
+    const int total = A.Yf * (A.zdf - A.GR - 1) * A.cpr_f + A.Yc * A.zdc * A.cpr_c + A.Yf * (A.zds - A.GR) * A.cpr_f;
+    const int nseg = (total + ENC_SEG - 1) / ENC_SEG;
+    const WsSegArgs W = {pl, pu, h->d.Yf, cpr, h->lower.nz};
+    const int nws = (W.Yf * W.cpr + WS_SEG - 1) / WS_SEG;
+    WsArgs Wf;
+    Wf.part = nullptr; Wf.Xf = h->d.Xf; Wf.Yf = h->d.Yf; Wf.NZs = h->d.NZs; Wf.GR = h->d.GR; Wf.omega_f = h->d.omega_f;
+    Wf.force = h->d_force; Wf.diag = diag_rec; Wf.out = nullptr; Wf.enc = nullptr;
+    Wf.halt = h->halt_armed ? h->halt_flag : nullptr;
+    hipLaunchKernelGGL(k_red_seg, dim3(nseg + nws), dim3(RB), 0, s, A, W, h->seg, nseg);
+    hipLaunchKernelGGL(k_fin_enc_ws, dim3(1), dim3(RB), 0, s, A, Wf, (const double *)h->seg, nseg, nws);
     return hipGetLastError();
 }
 
